@@ -1,0 +1,189 @@
+/*
+ * C-ABI of the gfx950 multi-evaluation library (libsgpp_hip.so).
+ *
+ * Plain pointers and sizes only. Every call returns an sgpp_hip_status;
+ * on failure sgpp_hip_last_error() returns a thread-local message. A C++
+ * wrapper maps the codes back to SG++'s exception types
+ * (base/src/sgpp/base/exception/).
+ *
+ * Array conventions: data points are M x d row-major FP64 (the layout of
+ * sgpp::base::DataMatrix); grid level/index arrays are N x d row-major
+ * uint32 in the grid storage's sequence order; all vectors are FP64.
+ */
+#ifndef SGPP_HIP_H_
+#define SGPP_HIP_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sgpp_hip_status {
+  SGPP_HIP_OK = 0,
+  SGPP_HIP_ERR_INVALID_ARGUMENT = 1, /* sgpp::base::data_exception, bad size/pointer */
+  SGPP_HIP_ERR_FACTORY = 2,          /* sgpp::base::factory_exception */
+  SGPP_HIP_ERR_OPERATION = 3,        /* sgpp::base::operation_exception */
+  SGPP_HIP_ERR_NOT_IMPLEMENTED = 4,  /* sgpp::base::not_implemented_exception */
+  SGPP_HIP_ERR_GENERATION = 5,       /* sgpp::base::generation_exception */
+  SGPP_HIP_ERR_DEVICE = 6,           /* HIP runtime failure */
+  SGPP_HIP_ERR_BUDGET = 7,           /* served loop: request exceeds the memory budget */
+  SGPP_HIP_ERR_NOT_FOUND = 8,        /* served loop: unknown model or request id */
+  SGPP_HIP_ERR_CANCELLED = 9,        /* served loop: request cancelled */
+  SGPP_HIP_ERR_TIMEOUT = 10,         /* served loop: request deadline passed */
+  SGPP_HIP_ERR_SOLVER = 11,          /* solver did not reach its tolerance */
+  SGPP_HIP_ERR_INTERNAL = 99
+} sgpp_hip_status;
+
+enum { SGPP_HIP_BASIS_LINEAR = 0, SGPP_HIP_BASIS_MODLINEAR = 1 };
+enum { SGPP_HIP_PREC_FP64 = 0, SGPP_HIP_PREC_FP32_FP64ACC = 1 };
+
+const char* sgpp_hip_last_error(void);
+int sgpp_hip_version(void);
+
+/* ---------------------------------------------------------------- grids
+ * Input builder with SG++-identical sequence numbers:
+ * Grid::createLinearGrid / createModLinearGrid (base/src/sgpp/base/grid/Grid.hpp:244,339),
+ * getGenerator().regular(level) (HashGenerator.hpp:282-342),
+ * getGenerator().refine(SurplusRefinementFunctor(alpha, n, thr)) (HashRefinement.cpp:124-141),
+ * OperationHierarchisation{Linear,ModLinear}::doHierarchisation.
+ */
+typedef struct sgpp_hip_grid sgpp_hip_grid;
+
+sgpp_hip_status sgpp_hip_grid_create(int basis, size_t dim, sgpp_hip_grid** out);
+sgpp_hip_status sgpp_hip_grid_destroy(sgpp_hip_grid* g);
+sgpp_hip_status sgpp_hip_grid_regular(sgpp_hip_grid* g, size_t level);
+size_t sgpp_hip_grid_size(const sgpp_hip_grid* g);
+size_t sgpp_hip_grid_dim(const sgpp_hip_grid* g);
+sgpp_hip_status sgpp_hip_grid_get_level_index(const sgpp_hip_grid* g, uint32_t* level,
+                                              uint32_t* index);
+sgpp_hip_status sgpp_hip_grid_get_coordinates(const sgpp_hip_grid* g, double* coords);
+sgpp_hip_status sgpp_hip_grid_get_leaf(const sgpp_hip_grid* g, uint8_t* leaf);
+/* surplus refinement of the refinements_num largest |alpha| with |alpha| >= threshold */
+sgpp_hip_status sgpp_hip_grid_refine_surplus(sgpp_hip_grid* g, const double* alpha,
+                                             size_t n_alpha, size_t refinements_num,
+                                             double threshold, size_t* n_added);
+/* full hierarchisation of node values (N in, N surpluses out) */
+sgpp_hip_status sgpp_hip_grid_hierarchise(sgpp_hip_grid* g, const double* node_values,
+                                          double* alpha);
+/* hierarchisation that reuses the per-sweep state of earlier calls on the
+ * same grid: only points appended since the last call are processed. The
+ * result is identical to sgpp_hip_grid_hierarchise. */
+sgpp_hip_status sgpp_hip_grid_hierarchise_incremental(sgpp_hip_grid* g,
+                                                      const double* node_values, double* alpha);
+/* Grid::serialize text ("linear"/"modlinear" + storage, version 9);
+ * *len receives the length incl. the terminating zero; buf may be NULL */
+sgpp_hip_status sgpp_hip_grid_serialize(const sgpp_hip_grid* g, char* buf, size_t cap,
+                                        size_t* len);
+sgpp_hip_status sgpp_hip_grid_unserialize(const char* text, sgpp_hip_grid** out);
+
+/* ------------------------------------------------------------- operator
+ * Replaces the operator a reference caller obtains from
+ * sgpp::op_factory::createOperationMultipleEval(grid, dataset, cfg)
+ * (datadriven/src/sgpp/datadriven/DatadrivenOpFactory.hpp:185-187) and its
+ * OperationMultipleEval interface (base/src/sgpp/base/operation/hash/
+ * OperationMultipleEval.hpp:27-130).
+ */
+typedef struct sgpp_hip_meval sgpp_hip_meval;
+
+typedef struct sgpp_hip_grid_desc {
+  int basis;  /* SGPP_HIP_BASIS_* */
+  size_t dim;
+} sgpp_hip_grid_desc;
+
+typedef struct sgpp_hip_params {
+  int precision;        /* SGPP_HIP_PREC_* used by mult / mult_transpose */
+  int keep_fp64;        /* also hold FP64 data when precision is FP32 */
+  int device;           /* HIP device ordinal */
+  int mult_grid_chunks; /* 0 = automatic */
+  int multT_data_segs;  /* 0 = automatic */
+  int target_blocks;    /* 0 = automatic */
+} sgpp_hip_params;
+
+/* constructor: copies, pads and transposes the dataset into HBM
+ * (OperationMultiEvalStreaming.cpp:13-25) */
+sgpp_hip_status sgpp_hip_meval_create(const sgpp_hip_grid_desc* grid, const double* data_rowmajor,
+                                      size_t M, size_t d, const sgpp_hip_params* params,
+                                      sgpp_hip_meval** out);
+/* operator bound to a grid handle, created through the C++ factory
+ * (createOperationMultipleEval with (STREAMING, HIP)); the grid must
+ * outlive the operator and prepare_grid() re-reads it after refinement */
+sgpp_hip_status sgpp_hip_meval_create_for_grid(sgpp_hip_grid* grid, const double* data_rowmajor,
+                                               size_t M, const sgpp_hip_params* params,
+                                               sgpp_hip_meval** out);
+/* prepare() (OperationMultiEvalStreaming.cpp:181): new grid arrays */
+sgpp_hip_status sgpp_hip_meval_prepare(sgpp_hip_meval* h, const uint32_t* level,
+                                       const uint32_t* index, size_t N);
+sgpp_hip_status sgpp_hip_meval_prepare_grid(sgpp_hip_meval* h);
+/* mult (OperationMultipleEval.hpp:55): alpha N -> result M (host pointers) */
+sgpp_hip_status sgpp_hip_meval_mult(sgpp_hip_meval* h, const double* alpha, double* result);
+/* multTranspose (OperationMultipleEval.hpp:81): source M -> result N */
+sgpp_hip_status sgpp_hip_meval_mult_transpose(sgpp_hip_meval* h, const double* source,
+                                              double* result);
+/* same on device pointers, asynchronous on the operator's stream */
+sgpp_hip_status sgpp_hip_meval_mult_device(sgpp_hip_meval* h, const double* d_alpha,
+                                           double* d_result, int precision);
+sgpp_hip_status sgpp_hip_meval_mult_transpose_device(sgpp_hip_meval* h, const double* d_source,
+                                                     double* d_result, int precision);
+sgpp_hip_status sgpp_hip_meval_synchronize(sgpp_hip_meval* h);
+/* getDuration() (OperationMultipleEval.hpp:120): seconds of the last host call */
+double sgpp_hip_meval_duration(const sgpp_hip_meval* h);
+/* kernel-only milliseconds of the last launch sequence (HIP events) */
+double sgpp_hip_meval_kernel_ms(sgpp_hip_meval* h);
+/* the operator's hipStream_t, as an opaque pointer */
+void* sgpp_hip_meval_stream(sgpp_hip_meval* h);
+size_t sgpp_hip_meval_num_data(const sgpp_hip_meval* h);
+size_t sgpp_hip_meval_num_grid(const sgpp_hip_meval* h);
+/* G and S chosen by the last mult / mult_transpose launch */
+sgpp_hip_status sgpp_hip_meval_last_shape(const sgpp_hip_meval* h, int* mult_chunks,
+                                          int* multT_segs);
+/* HIP-event timing of the main mult / multTranspose kernel of every launch
+ * on the operator's stream (prep and reduction kernels excluded):
+ * enable (resets the totals), then read count and total milliseconds. */
+sgpp_hip_status sgpp_hip_meval_kernel_timing_enable(sgpp_hip_meval* h, int on);
+sgpp_hip_status sgpp_hip_meval_kernel_timing(sgpp_hip_meval* h, int* n_mult, double* ms_mult,
+                                             int* n_multT, double* ms_multT);
+sgpp_hip_status sgpp_hip_meval_destroy(sgpp_hip_meval* h);
+
+/* --------------------------------------------------------------- solver
+ * (B^T B + lambda * M * I) alpha = B^T y, the system of
+ * SystemMatrixLeastSquaresIdentity (datadriven/src/sgpp/datadriven/algorithm/
+ * SystemMatrixLeastSquaresIdentity.cpp:32-56) solved by
+ * ConjugateGradients::solve (solver/src/sgpp/solver/sle/ConjugateGradients.cpp:22-151).
+ */
+enum { SGPP_HIP_SOLVE_FP64 = 0, SGPP_HIP_SOLVE_MIXED = 1 };
+
+typedef struct sgpp_hip_solve_params {
+  int mode;                 /* SGPP_HIP_SOLVE_* */
+  double lambda;            /* regularisation lambda (system uses M * lambda) */
+  size_t max_iterations;    /* CG iterations (per inner solve in mixed mode) */
+  double epsilon;           /* stop when ||r|| <= epsilon * ||r0|| (FP64 residual) */
+  double inner_epsilon;     /* mixed mode: inner CG reduction per refinement step */
+  size_t max_refinements;   /* mixed mode: outer iterative-refinement steps */
+  int reuse;                /* start from the alpha passed in (FP64 mode) */
+} sgpp_hip_solve_params;
+
+typedef struct sgpp_hip_solve_info {
+  size_t iterations;        /* CG iterations (all inner iterations in mixed mode) */
+  size_t refinements;       /* outer refinement steps (mixed mode) */
+  size_t fp64_matvecs;      /* applications of A in FP64 */
+  size_t fp32_matvecs;      /* applications of A in FP32 */
+  double residual_rel;      /* final ||b - A alpha|| / ||b|| in FP64 */
+  double seconds;           /* wall-clock of the solve */
+} sgpp_hip_solve_info;
+
+/* y: M values, alpha: N values (in: start value when reuse, out: solution).
+ * The operator must serve FP64 (FP64 mode) or FP32 and FP64 (mixed mode). */
+sgpp_hip_status sgpp_hip_regression_solve(sgpp_hip_meval* h, const double* y,
+                                          const sgpp_hip_solve_params* p, double* alpha,
+                                          sgpp_hip_solve_info* info);
+/* b = B^T y (SystemMatrixLeastSquaresIdentity::generateb) and q = A v, FP64 */
+sgpp_hip_status sgpp_hip_system_matrix_mult(sgpp_hip_meval* h, double lambda, const double* v,
+                                            double* q, int precision);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGPP_HIP_H_ */

@@ -1,0 +1,274 @@
+"""ORACLE -- test infrastructure only.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this package, and only as the checker or the CPU baseline; the product
+(sgpp_amd) never imports it.
+
+Two layers:
+  * ``_oracle.so`` (sgpp_oracle.c): plain-C restatement of SG++'s operator
+    definition (LinearBasis / LinearModifiedBasis products summed in long
+    double) and of the StreamingDefault CPU operators ("port").
+  * ``_ref/libsgpp_ref_v{3,4}.so``: the reference's own grid, DEFAULT,
+    StreamingDefault, SubspaceLinear and CG code, compiled from its sources by
+    oracle/Makefile (plus our thin ref_driver.cpp). Present wherever it was
+    built; the v4 (AVX-512) build is used only on hosts whose CPU reports
+    the AVX-512 subset it needs.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_dp = ctypes.POINTER(ctypes.c_double)
+_up = ctypes.POINTER(ctypes.c_uint32)
+_sz = ctypes.c_size_t
+_vp = ctypes.c_void_p
+
+BASIS_LINEAR, BASIS_MODLINEAR = 0, 1
+KIND_DEFAULT, KIND_STREAMING, KIND_SUBSPACE = 0, 1, 2
+
+_lib = None
+_ref = None
+_ref_variant = None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _u(a):
+    return a.ctypes.data_as(_up)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "_oracle.so")
+        if not os.path.exists(path):
+            raise ImportError("oracle/_oracle.so is not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.oracle_mult.argtypes = [ctypes.c_int, _sz, _sz, _dp, _sz, _up, _up, _dp, _dp]
+        L.oracle_mult_transpose.argtypes = [ctypes.c_int, _sz, _sz, _dp, _sz, _up, _up, _dp, _dp]
+        L.oracle_phi.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double]
+        L.oracle_phi.restype = ctypes.c_double
+        L.port_create.argtypes = [ctypes.c_int, _sz, _sz, _dp, _sz, _up, _up]
+        L.port_create.restype = _vp
+        L.port_destroy.argtypes = [_vp]
+        L.port_mult.argtypes = [_vp, _dp, _dp]
+        L.port_mult_transpose.argtypes = [_vp, _dp, _dp]
+        L.oracle_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def phi(basis, level, index, x):
+    return lib().oracle_phi(basis, level, index, x)
+
+
+def mult(basis, X, level, index, alpha):
+    """r = B alpha by the definition (long double sums)."""
+    X, lv, ix, a = _f64(X), _u32(level), _u32(index), _f64(alpha)
+    M, d = X.shape
+    r = np.empty(M)
+    lib().oracle_mult(basis, d, M, _d(X), lv.shape[0], _u(lv), _u(ix), _d(a), _d(r))
+    return r
+
+
+def mult_transpose(basis, X, level, index, y):
+    """a = B^T y by the definition (long double sums)."""
+    X, lv, ix, yy = _f64(X), _u32(level), _u32(index), _f64(y)
+    M, d = X.shape
+    a = np.empty(lv.shape[0])
+    lib().oracle_mult_transpose(basis, d, M, _d(X), lv.shape[0], _u(lv), _u(ix), _d(yy), _d(a))
+    return a
+
+
+class Port:
+    """StreamingDefault restatement (FP64, OpenMP), the 'port' CPU baseline."""
+
+    def __init__(self, basis, X, level, index):
+        X, lv, ix = _f64(X), _u32(level), _u32(index)
+        self.M, self.d = X.shape
+        self.N = lv.shape[0]
+        self.h = lib().port_create(basis, self.d, self.M, _d(X), self.N, _u(lv), _u(ix))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().port_destroy(self.h)
+            self.h = None
+
+    def mult(self, alpha):
+        a = _f64(alpha)
+        r = np.empty(self.M)
+        lib().port_mult(self.h, _d(a), _d(r))
+        return r
+
+    def mult_transpose(self, y):
+        yy = _f64(y)
+        a = np.empty(self.N)
+        lib().port_mult_transpose(self.h, _d(yy), _d(a))
+        return a
+
+
+def num_threads():
+    return lib().oracle_num_threads()
+
+
+# ------------------------------------------------------------ reference build
+
+def _cpu_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as f:
+            flags = set()
+            for line in f:
+                if line.startswith("flags"):
+                    flags = set(line.split(":", 1)[1].split())
+                    break
+        return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        return False
+
+
+def ref_variant():
+    ref()
+    return _ref_variant
+
+
+def ref():
+    """The reference build, or None where it was not built."""
+    global _ref, _ref_variant
+    if _ref is not None:
+        return _ref
+    variants = (["v4", "v3"] if _cpu_has_avx512() else ["v3"])
+    for v in variants:
+        path = os.path.join(HERE, "_ref", "libsgpp_ref_%s.so" % v)
+        if os.path.exists(path):
+            # lazy binding: the reference library has a few unresolved symbols of
+            # basis types that are never called on this path
+            R = ctypes.CDLL(path, mode=os.RTLD_LAZY | os.RTLD_LOCAL)
+            R.ref_last_error.restype = ctypes.c_char_p
+            R.ref_grid_new.argtypes = [ctypes.c_int, _sz]
+            R.ref_grid_new.restype = _vp
+            R.ref_grid_free.argtypes = [_vp]
+            R.ref_grid_regular.argtypes = [_vp, _sz]
+            R.ref_grid_size.argtypes = [_vp]
+            R.ref_grid_size.restype = _sz
+            R.ref_grid_level_index.argtypes = [_vp, _up, _up, ctypes.POINTER(ctypes.c_uint8)]
+            R.ref_grid_refine_surplus.argtypes = [_vp, _dp, _sz, _sz, ctypes.c_double]
+            R.ref_grid_hierarchise.argtypes = [_vp, _dp, _dp]
+            R.ref_op_new.argtypes = [_vp, ctypes.c_int, _dp, _sz]
+            R.ref_op_new.restype = _vp
+            R.ref_op_free.argtypes = [_vp]
+            R.ref_op_prepare.argtypes = [_vp]
+            R.ref_op_mult.argtypes = [_vp, _dp, _sz, _dp, _sz]
+            R.ref_op_mult_transpose.argtypes = [_vp, _dp, _sz, _dp, _sz]
+            R.ref_op_seconds.argtypes = [_vp]
+            R.ref_op_seconds.restype = ctypes.c_double
+            R.ref_cg_solve.argtypes = [_vp, _sz, _sz, _dp, ctypes.c_double, _sz, ctypes.c_double,
+                                       _dp, ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)]
+            _ref, _ref_variant = R, v
+            return R
+    return None
+
+
+def _rcheck(rc):
+    if rc != 0:
+        raise RuntimeError("reference error: " + ref().ref_last_error().decode())
+
+
+class RefGrid:
+    """The reference's own Grid (Linear / ModLinear)."""
+
+    def __init__(self, basis, dim):
+        R = ref()
+        if R is None:
+            raise RuntimeError("reference build not available")
+        self.basis, self.dim = basis, dim
+        self.h = R.ref_grid_new(basis, dim)
+
+    def __del__(self):
+        if getattr(self, "h", None) and ref() is not None:
+            ref().ref_grid_free(self.h)
+            self.h = None
+
+    @property
+    def size(self):
+        return ref().ref_grid_size(self.h)
+
+    def regular(self, level):
+        _rcheck(ref().ref_grid_regular(self.h, level))
+        return self
+
+    def level_index(self, with_leaf=False):
+        n = self.size
+        lv = np.empty((n, self.dim), dtype=np.uint32)
+        ix = np.empty((n, self.dim), dtype=np.uint32)
+        lf = np.empty(n, dtype=np.uint8)
+        _rcheck(ref().ref_grid_level_index(self.h, _u(lv), _u(ix),
+                                           lf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return (lv, ix, lf) if with_leaf else (lv, ix)
+
+    def refine_surplus(self, alpha, num, threshold=0.0):
+        a = _f64(alpha)
+        _rcheck(ref().ref_grid_refine_surplus(self.h, _d(a), a.size, num, threshold))
+
+    def hierarchise(self, values):
+        v = _f64(values)
+        out = np.empty_like(v)
+        _rcheck(ref().ref_grid_hierarchise(self.h, _d(v), _d(out)))
+        return out
+
+
+class RefOp:
+    """A reference OperationMultipleEval (DEFAULT / StreamingDefault / SubspaceLinear)."""
+
+    def __init__(self, grid, kind, X):
+        X = _f64(X)
+        self.grid = grid
+        self.M = X.shape[0]
+        self.h = ref().ref_op_new(grid.h, kind, _d(X), self.M)
+        if not self.h:
+            raise RuntimeError("reference error: " + ref().ref_last_error().decode())
+
+    def __del__(self):
+        if getattr(self, "h", None) and ref() is not None:
+            ref().ref_op_free(self.h)
+            self.h = None
+
+    def prepare(self):
+        _rcheck(ref().ref_op_prepare(self.h))
+
+    def mult(self, alpha):
+        a = _f64(alpha)
+        r = np.empty(self.M)
+        _rcheck(ref().ref_op_mult(self.h, _d(a), a.size, _d(r), self.M))
+        return r
+
+    def mult_transpose(self, y):
+        yy = _f64(y)
+        n = self.grid.size
+        a = np.empty(n)
+        _rcheck(ref().ref_op_mult_transpose(self.h, _d(yy), self.M, _d(a), n))
+        return a
+
+    def seconds(self):
+        return ref().ref_op_seconds(self.h)
+
+    def cg_solve(self, y, lam, max_it, eps):
+        yy = _f64(y)
+        n = self.grid.size
+        a = np.empty(n)
+        it = _sz(0)
+        res = ctypes.c_double(0)
+        _rcheck(ref().ref_cg_solve(self.h, self.M, n, _d(yy), lam, max_it, eps, _d(a),
+                                   ctypes.byref(it), ctypes.byref(res)))
+        return a, it.value, res.value

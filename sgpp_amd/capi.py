@@ -1,0 +1,347 @@
+"""ctypes binding of libsgpp_hip.so (declarations: include/sgpp_hip.h).
+
+Python mirror of the C++ operator interface for tests and benchmarks:
+``Grid`` (createLinearGrid / createModLinearGrid, regular, refine,
+hierarchise), ``MultiEval`` (OperationMultipleEval: mult, multTranspose,
+prepare, getDuration) and ``regression_solve``. There is no fallback: if the
+library is missing, importing this module raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsgpp_hip.so")
+
+OK = 0
+STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "FACTORY", 3: "OPERATION", 4: "NOT_IMPLEMENTED",
+                5: "GENERATION", 6: "DEVICE", 7: "BUDGET", 8: "NOT_FOUND", 9: "CANCELLED",
+                10: "TIMEOUT", 11: "SOLVER", 99: "INTERNAL"}
+BASIS_LINEAR, BASIS_MODLINEAR = 0, 1
+PREC_FP64, PREC_FP32 = 0, 1
+SOLVE_FP64, SOLVE_MIXED = 0, 1
+
+
+class SGppHIPError(RuntimeError):
+    def __init__(self, status, message):
+        self.status = status
+        self.status_name = STATUS_NAMES.get(status, str(status))
+        super().__init__("%s: %s" % (self.status_name, message))
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libsgpp_hip.so is not built (run __graft_entry__.build() or "
+                      "python -m sgpp_amd.build); there is no CPU fallback")
+
+# One HIP runtime per process: PyTorch ships its own libamdhip64 (loaded by
+# the unversioned name), the library needs libamdhip64.so.7. Loading torch
+# first makes the library bind to torch's copy instead of loading a second
+# runtime, which would then fail to find the device.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for the C-ABI
+    pass
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_p = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_st = ctypes.c_int
+_dp = ctypes.POINTER(ctypes.c_double)
+_up = ctypes.POINTER(ctypes.c_uint32)
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("basis", ctypes.c_int), ("dim", ctypes.c_size_t)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("keep_fp64", ctypes.c_int), ("device", ctypes.c_int),
+                ("mult_grid_chunks", ctypes.c_int), ("multT_data_segs", ctypes.c_int),
+                ("target_blocks", ctypes.c_int)]
+
+
+class SolveParams(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int), ("lambda_", ctypes.c_double),
+                ("max_iterations", ctypes.c_size_t), ("epsilon", ctypes.c_double),
+                ("inner_epsilon", ctypes.c_double), ("max_refinements", ctypes.c_size_t),
+                ("reuse", ctypes.c_int)]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_size_t), ("refinements", ctypes.c_size_t),
+                ("fp64_matvecs", ctypes.c_size_t), ("fp32_matvecs", ctypes.c_size_t),
+                ("residual_rel", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+def _decl(name, restype, argtypes):
+    f = getattr(lib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    return f
+
+
+# every symbol declared in include/sgpp_hip.h, with its ctypes signature
+SIGNATURES = {
+    "sgpp_hip_last_error": (ctypes.c_char_p, []),
+    "sgpp_hip_version": (ctypes.c_int, []),
+    "sgpp_hip_grid_create": (_st, [ctypes.c_int, _sz, ctypes.POINTER(_p)]),
+    "sgpp_hip_grid_destroy": (_st, [_p]),
+    "sgpp_hip_grid_regular": (_st, [_p, _sz]),
+    "sgpp_hip_grid_size": (_sz, [_p]),
+    "sgpp_hip_grid_dim": (_sz, [_p]),
+    "sgpp_hip_grid_get_level_index": (_st, [_p, _up, _up]),
+    "sgpp_hip_grid_get_coordinates": (_st, [_p, _dp]),
+    "sgpp_hip_grid_get_leaf": (_st, [_p, ctypes.POINTER(ctypes.c_uint8)]),
+    "sgpp_hip_grid_refine_surplus": (_st, [_p, _dp, _sz, _sz, ctypes.c_double,
+                                           ctypes.POINTER(_sz)]),
+    "sgpp_hip_grid_hierarchise": (_st, [_p, _dp, _dp]),
+    "sgpp_hip_grid_hierarchise_incremental": (_st, [_p, _dp, _dp]),
+    "sgpp_hip_grid_serialize": (_st, [_p, ctypes.c_char_p, _sz, ctypes.POINTER(_sz)]),
+    "sgpp_hip_grid_unserialize": (_st, [ctypes.c_char_p, ctypes.POINTER(_p)]),
+    "sgpp_hip_meval_create": (_st, [ctypes.POINTER(GridDesc), _dp, _sz, _sz,
+                                    ctypes.POINTER(Params), ctypes.POINTER(_p)]),
+    "sgpp_hip_meval_create_for_grid": (_st, [_p, _dp, _sz, ctypes.POINTER(Params),
+                                             ctypes.POINTER(_p)]),
+    "sgpp_hip_meval_prepare": (_st, [_p, _up, _up, _sz]),
+    "sgpp_hip_meval_prepare_grid": (_st, [_p]),
+    "sgpp_hip_meval_mult": (_st, [_p, _dp, _dp]),
+    "sgpp_hip_meval_mult_transpose": (_st, [_p, _dp, _dp]),
+    "sgpp_hip_meval_mult_device": (_st, [_p, _p, _p, ctypes.c_int]),
+    "sgpp_hip_meval_mult_transpose_device": (_st, [_p, _p, _p, ctypes.c_int]),
+    "sgpp_hip_meval_synchronize": (_st, [_p]),
+    "sgpp_hip_meval_duration": (ctypes.c_double, [_p]),
+    "sgpp_hip_meval_kernel_ms": (ctypes.c_double, [_p]),
+    "sgpp_hip_meval_stream": (_p, [_p]),
+    "sgpp_hip_meval_num_data": (_sz, [_p]),
+    "sgpp_hip_meval_num_grid": (_sz, [_p]),
+    "sgpp_hip_meval_last_shape": (_st, [_p, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]),
+    "sgpp_hip_meval_kernel_timing_enable": (_st, [_p, ctypes.c_int]),
+    "sgpp_hip_meval_kernel_timing": (_st, [_p, ctypes.POINTER(ctypes.c_int), _dp,
+                                           ctypes.POINTER(ctypes.c_int), _dp]),
+    "sgpp_hip_meval_destroy": (_st, [_p]),
+    "sgpp_hip_regression_solve": (_st, [_p, _dp, ctypes.POINTER(SolveParams), _dp,
+                                        ctypes.POINTER(SolveInfo)]),
+    "sgpp_hip_system_matrix_mult": (_st, [_p, ctypes.c_double, _dp, _dp, ctypes.c_int]),
+}
+
+F = {}
+for _name, (_res, _args) in SIGNATURES.items():
+    F[_name] = _decl(_name, _res, _args)
+
+
+def last_error():
+    return F["sgpp_hip_last_error"]().decode(errors="replace")
+
+
+def check(status):
+    if status != OK:
+        raise SGppHIPError(status, last_error())
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Grid:
+    """Linear / ModLinear no-boundary sparse grid with SG++ sequence numbers."""
+
+    def __init__(self, basis, dim, _handle=None):
+        self.basis = basis
+        if _handle is not None:
+            self.h = _handle
+        else:
+            h = _p()
+            check(F["sgpp_hip_grid_create"](basis, dim, ctypes.byref(h)))
+            self.h = h
+
+    @classmethod
+    def linear(cls, dim):
+        return cls(BASIS_LINEAR, dim)
+
+    @classmethod
+    def modlinear(cls, dim):
+        return cls(BASIS_MODLINEAR, dim)
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and F is not None:
+            F["sgpp_hip_grid_destroy"](h)
+            self.h = None
+
+    @property
+    def size(self):
+        return F["sgpp_hip_grid_size"](self.h)
+
+    @property
+    def dim(self):
+        return F["sgpp_hip_grid_dim"](self.h)
+
+    def regular(self, level):
+        check(F["sgpp_hip_grid_regular"](self.h, level))
+        return self
+
+    def level_index(self):
+        n, d = self.size, self.dim
+        lv = np.empty((n, d), dtype=np.uint32)
+        ix = np.empty((n, d), dtype=np.uint32)
+        check(F["sgpp_hip_grid_get_level_index"](self.h, lv.ctypes.data_as(_up),
+                                                 ix.ctypes.data_as(_up)))
+        return lv, ix
+
+    def coordinates(self):
+        c = np.empty((self.size, self.dim), dtype=np.float64)
+        check(F["sgpp_hip_grid_get_coordinates"](self.h, _dptr(c)))
+        return c
+
+    def leaf(self):
+        a = np.empty(self.size, dtype=np.uint8)
+        check(F["sgpp_hip_grid_get_leaf"](self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))))
+        return a
+
+    def refine_surplus(self, alpha, refinements_num, threshold=0.0):
+        a = _f64(alpha)
+        n = _sz(0)
+        check(F["sgpp_hip_grid_refine_surplus"](self.h, _dptr(a), a.size, refinements_num,
+                                                threshold, ctypes.byref(n)))
+        return n.value
+
+    def hierarchise(self, node_values, incremental=False):
+        v = _f64(node_values)
+        out = np.empty_like(v)
+        fn = "sgpp_hip_grid_hierarchise_incremental" if incremental else "sgpp_hip_grid_hierarchise"
+        check(F[fn](self.h, _dptr(v), _dptr(out)))
+        return out
+
+    def serialize(self):
+        n = _sz(0)
+        check(F["sgpp_hip_grid_serialize"](self.h, None, 0, ctypes.byref(n)))
+        buf = ctypes.create_string_buffer(n.value)
+        check(F["sgpp_hip_grid_serialize"](self.h, buf, n.value, ctypes.byref(n)))
+        return buf.value.decode()
+
+    @classmethod
+    def unserialize(cls, text):
+        h = _p()
+        check(F["sgpp_hip_grid_unserialize"](text.encode(), ctypes.byref(h)))
+        basis = BASIS_LINEAR if text.lstrip().startswith("linear") else BASIS_MODLINEAR
+        return cls(basis, 0, _handle=h)
+
+
+class MultiEval:
+    """OperationMultipleEval on the GPU (mult = B alpha, multTranspose = B^T y)."""
+
+    def __init__(self, basis, data, precision=PREC_FP64, keep_fp64=False, device=0,
+                 mult_grid_chunks=0, multT_data_segs=0, target_blocks=0, grid=None):
+        data = _f64(data)
+        if data.ndim != 2:
+            raise ValueError("data must be M x d")
+        self.M, self.d = data.shape
+        self.precision = precision
+        prm = Params(precision, int(keep_fp64), device, mult_grid_chunks, multT_data_segs,
+                     target_blocks)
+        h = _p()
+        if grid is not None:
+            self.grid = grid  # keep alive
+            check(F["sgpp_hip_meval_create_for_grid"](grid.h, _dptr(data), self.M,
+                                                      ctypes.byref(prm), ctypes.byref(h)))
+        else:
+            desc = GridDesc(basis, self.d)
+            check(F["sgpp_hip_meval_create"](ctypes.byref(desc), _dptr(data), self.M, self.d,
+                                             ctypes.byref(prm), ctypes.byref(h)))
+        self.h = h
+        self.N = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and F is not None:
+            F["sgpp_hip_meval_destroy"](h)
+            self.h = None
+
+    def prepare(self, level=None, index=None):
+        if level is None:
+            check(F["sgpp_hip_meval_prepare_grid"](self.h))
+        else:
+            lv = np.ascontiguousarray(level, dtype=np.uint32)
+            ix = np.ascontiguousarray(index, dtype=np.uint32)
+            if lv.shape != ix.shape or lv.ndim != 2 or lv.shape[1] != self.d:
+                raise ValueError("level/index must both be N x d")
+            check(F["sgpp_hip_meval_prepare"](self.h, lv.ctypes.data_as(_up),
+                                              ix.ctypes.data_as(_up), lv.shape[0]))
+        self.N = F["sgpp_hip_meval_num_grid"](self.h)
+        return self
+
+    def mult(self, alpha):
+        a = _f64(alpha)
+        if a.size != self.N:
+            raise ValueError("alpha must have N entries")
+        r = np.empty(self.M, dtype=np.float64)
+        check(F["sgpp_hip_meval_mult"](self.h, _dptr(a), _dptr(r)))
+        return r
+
+    def mult_transpose(self, source):
+        y = _f64(source)
+        if y.size != self.M:
+            raise ValueError("source must have M entries")
+        a = np.empty(self.N, dtype=np.float64)
+        check(F["sgpp_hip_meval_mult_transpose"](self.h, _dptr(y), _dptr(a)))
+        return a
+
+    def mult_device(self, d_alpha_ptr, d_result_ptr, precision=None):
+        p = self.precision if precision is None else precision
+        check(F["sgpp_hip_meval_mult_device"](self.h, d_alpha_ptr, d_result_ptr, p))
+
+    def mult_transpose_device(self, d_source_ptr, d_result_ptr, precision=None):
+        p = self.precision if precision is None else precision
+        check(F["sgpp_hip_meval_mult_transpose_device"](self.h, d_source_ptr, d_result_ptr, p))
+
+    def synchronize(self):
+        check(F["sgpp_hip_meval_synchronize"](self.h))
+
+    def duration(self):
+        return F["sgpp_hip_meval_duration"](self.h)
+
+    def kernel_ms(self):
+        return F["sgpp_hip_meval_kernel_ms"](self.h)
+
+    def stream(self):
+        return F["sgpp_hip_meval_stream"](self.h)
+
+    def last_shape(self):
+        g, s = ctypes.c_int(0), ctypes.c_int(0)
+        check(F["sgpp_hip_meval_last_shape"](self.h, ctypes.byref(g), ctypes.byref(s)))
+        return g.value, s.value
+
+    def kernel_timing_enable(self, on=True):
+        check(F["sgpp_hip_meval_kernel_timing_enable"](self.h, int(on)))
+
+    def kernel_timing(self):
+        """(n_mult, ms_mult, n_multT, ms_multT) of the main kernels since enabling."""
+        a, c = ctypes.c_int(0), ctypes.c_int(0)
+        b, d = ctypes.c_double(0), ctypes.c_double(0)
+        check(F["sgpp_hip_meval_kernel_timing"](self.h, ctypes.byref(a), ctypes.byref(b),
+                                                ctypes.byref(c), ctypes.byref(d)))
+        return a.value, b.value, c.value, d.value
+
+    def system_mult(self, v, lam, precision=PREC_FP64):
+        v = _f64(v)
+        q = np.empty_like(v)
+        check(F["sgpp_hip_system_matrix_mult"](self.h, lam, _dptr(v), _dptr(q), precision))
+        return q
+
+    def solve(self, y, lam, mode=SOLVE_FP64, max_iterations=1000, epsilon=1e-10,
+              inner_epsilon=1e-5, max_refinements=20, alpha0=None):
+        y = _f64(y)
+        alpha = np.zeros(self.N) if alpha0 is None else _f64(alpha0).copy()
+        prm = SolveParams(mode, lam, max_iterations, epsilon, inner_epsilon, max_refinements,
+                          int(alpha0 is not None))
+        info = SolveInfo()
+        check(F["sgpp_hip_regression_solve"](self.h, _dptr(y), ctypes.byref(prm), _dptr(alpha),
+                                             ctypes.byref(info)))
+        return alpha, {k: getattr(info, k) for k, _ in SolveInfo._fields_}

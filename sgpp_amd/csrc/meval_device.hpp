@@ -1,0 +1,152 @@
+// Device side of the multi-evaluation operator: buffers, launches, timing.
+//
+// The operator evaluates B (M x N, B_ij = phi_j(x_i)) on a sparse grid with
+// the Linear or ModLinear hat basis:
+//   mult:          r = B * alpha   (grid -> data)
+//   multTranspose: a = B^T * y     (data -> grid)
+// Everything here is owned by one DeviceMultiEval object; the C++ operator
+// class (OperationMultiEvalHIP) and the C-ABI sit on top of it.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sgpp_hip {
+
+enum class Basis : int { Linear = 0, ModLinear = 1 };
+enum class Precision : int { FP64 = 0, FP32 = 1 };
+
+// Error thrown for any failing HIP call or invalid launch shape.
+struct device_error : public std::runtime_error {
+  explicit device_error(const std::string& m) : std::runtime_error(m) {}
+};
+
+// Error thrown for inputs the operator cannot evaluate (bad sizes, data
+// outside the unit cube, level/index pairs that are not no-boundary points).
+struct input_error : public std::runtime_error {
+  explicit input_error(const std::string& m) : std::runtime_error(m) {}
+};
+
+void check_hip(hipError_t e, const char* what);
+
+// Launch tuning knobs; 0 means "choose automatically".
+struct LaunchConfig {
+  int mult_grid_chunks = 0;   // G: grid-point chunks for mult (partials over G)
+  int multT_data_segs = 0;    // S: data segments for multTranspose (partials over S)
+  int target_blocks = 0;      // workgroups to aim for per launch
+};
+
+// Encodes level/index pairs of one grid into the (L, I) pair per point and
+// dimension that the kernels consume, plus a per-point power-of-two scale:
+//   Linear:    phi = max(0, 1 - |L x - I|),      L = 2^l, I = i, scale 1
+//   ModLinear: level 1            -> (0, 0)       (phi == 1)
+//              i == 1             -> (2^l/2, 0)   (phi/2 = 1 - 2^l x / 2)
+//              i == 2^l - 1       -> (2^l/2, 2^l/2)
+//              otherwise          -> (2^l, i)
+//              scale = 2^(number of modified boundary factors)
+// Both bases then share the kernel step p <- clamp(p - p*|L x - I|).
+void encode_grid(Basis basis, size_t dim, size_t n, const uint32_t* level, const uint32_t* index,
+                 std::vector<double>& li, std::vector<double>& scale);
+
+class DeviceMultiEval {
+ public:
+  // data_rowmajor: M x d, FP64 row-major (the caller's DataMatrix layout).
+  // precisions: bitmask of (1 << Precision) that the instance must serve.
+  DeviceMultiEval(const double* data_rowmajor, size_t M, size_t dim, Basis basis,
+                  unsigned precisions, int device, const LaunchConfig& cfg);
+  ~DeviceMultiEval();
+  DeviceMultiEval(const DeviceMultiEval&) = delete;
+  DeviceMultiEval& operator=(const DeviceMultiEval&) = delete;
+
+  // level/index: N x d row-major uint32 (the grid storage's point order).
+  void set_grid(const uint32_t* level, const uint32_t* index, size_t N);
+
+  // Device-pointer entry points (inputs/outputs already in HBM, FP64).
+  // alpha: N values, result: M values; y: M values, result: N values.
+  void mult_device(const double* d_alpha, double* d_result, Precision p);
+  void mult_transpose_device(const double* d_y, double* d_result, Precision p);
+
+  // Host-pointer entry points (copy in, compute, copy out, synchronise).
+  void mult_host(const double* alpha, double* result, Precision p);
+  void mult_transpose_host(const double* y, double* result, Precision p);
+
+  void synchronize();
+  hipStream_t stream() const { return stream_; }
+  size_t num_data() const { return M_; }
+  size_t num_data_padded() const { return Mpad_; }
+  size_t num_grid() const { return N_; }
+  size_t dim() const { return D_; }
+  Basis basis() const { return basis_; }
+  bool serves(Precision p) const { return (precisions_ >> static_cast<int>(p)) & 1u; }
+
+  // Kernel-only milliseconds of the last mult / multTranspose (HIP events).
+  double last_kernel_ms() const { return last_kernel_ms_; }
+  // Wall-clock seconds of the last host-pointer call (copies included).
+  double last_wall_s() const { return last_wall_s_; }
+
+  // Shapes the last launches used (for reporting).
+  int last_mult_chunks() const { return last_G_; }
+  int last_multT_segs() const { return last_S_; }
+
+  // HIP-event timing of the main mult / multTranspose kernel of every launch
+  // (prep and reduction kernels excluded), on this operator's stream.
+  // Enabling resets the totals; timing() synchronises and sums the launches
+  // since then.
+  void set_kernel_timing(bool on);
+  void kernel_timing(int* n_mult, double* ms_mult, int* n_multT, double* ms_multT);
+
+ private:
+  void ensure(void** buf, size_t* cap, size_t bytes);
+  void record_start();
+  void record_stop();
+
+  size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
+  Basis basis_;
+  unsigned precisions_;
+  int device_ = 0;
+  LaunchConfig cfg_;
+  int num_cus_ = 256;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+
+  // data, SoA dim-major [D][Mpad]
+  void* X64_ = nullptr;
+  void* X32_ = nullptr;
+  // grid pairs [Npad][D][2] and per-point scale [Npad]
+  void* LI64_ = nullptr;
+  void* LI32_ = nullptr;
+  double* scale_ = nullptr;
+  size_t gridcap_li64_ = 0, gridcap_li32_ = 0, gridcap_scale_ = 0;
+
+  // scratch
+  void* A_ = nullptr;     size_t A_cap_ = 0;      // alpha*scale in compute type
+  void* Y_ = nullptr;     size_t Y_cap_ = 0;      // padded y in compute type
+  void* part_ = nullptr;  size_t part_cap_ = 0;   // FP64 partials
+  void* hin_ = nullptr;   size_t hin_cap_ = 0;    // device staging for host calls
+  void* hout_ = nullptr;  size_t hout_cap_ = 0;
+
+  struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  EventPair* next_timer(std::vector<EventPair>& pool, size_t& used);
+  bool timing_on_ = false;
+  std::vector<EventPair> tm_mult_, tm_multT_;
+  size_t used_mult_ = 0, used_multT_ = 0;
+
+  double last_kernel_ms_ = 0.0;
+  double last_wall_s_ = 0.0;
+  int last_G_ = 0, last_S_ = 0;
+};
+
+// Padding units (points): data is padded to a multiple of kDataPad by
+// repeating the last row, the grid to a multiple of kGridPad with
+// points that evaluate to a constant and carry zero weight.
+constexpr size_t kDataPad = 4096;
+constexpr size_t kGridPad = 2048;
+
+}  // namespace sgpp_hip

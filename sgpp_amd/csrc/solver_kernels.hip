@@ -1,0 +1,291 @@
+// Device-resident least-squares solve for (B^T B + lambda*M*I) alpha = B^T y.
+//
+// FP64 mode follows ConjugateGradients::solve
+// (solver/src/sgpp/solver/sle/ConjugateGradients.cpp:22-151) step by step:
+// start alpha = 0 unless reused, stop when delta_new <= eps^2 * delta_0,
+// recompute the true residual every 50 iterations, break on d.q == 0.
+// The system matrix is SystemMatrixLeastSquaresIdentity::mult
+// (datadriven/src/sgpp/datadriven/algorithm/SystemMatrixLeastSquaresIdentity.cpp:32-47):
+// temp = B v, q = B^T temp, q += M*lambda*v.
+//
+// Mixed mode: FP64 iterative refinement around FP32-matvec CG. Vectors, dot
+// products and the residual r = b - A alpha are FP64; only the products
+// inside the inner CG use the FP32-with-FP64-accumulation kernels:
+//   r = b - A64 alpha; repeat { solve A32 c = r to inner_eps (CG, FP64
+//   vectors); alpha += c; r = b - A64 alpha } until ||r|| <= eps ||b||.
+// Dot products use a fixed two-stage reduction (deterministic).
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "meval_device.hpp"
+#include "solver_device.hpp"
+
+namespace sgpp_hip {
+
+namespace {
+
+constexpr int kRB = 256;
+constexpr int kDotBlocks = 256;
+
+__global__ void axpy_kernel(double a, const double* __restrict__ x, double* __restrict__ y,
+                            int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += a * x[i];
+}
+// y = x + b*y
+__global__ void xpby_kernel(const double* __restrict__ x, double b, double* __restrict__ y,
+                            int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + b * y[i];
+}
+// z = x - y
+__global__ void sub_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                           double* __restrict__ z, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = x[i] - y[i];
+}
+__global__ void fill_kernel(double* __restrict__ y, double v, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+// stage 1: block b sums the contiguous chunk [b*chunk, (b+1)*chunk)
+__global__ __launch_bounds__(kRB) void dot_stage1(const double* __restrict__ x,
+                                                  const double* __restrict__ y, int64_t n,
+                                                  int64_t chunk, double* __restrict__ part) {
+  __shared__ double s[kRB];
+  const int64_t b0 = static_cast<int64_t>(blockIdx.x) * chunk;
+  const int64_t b1 = min(b0 + chunk, n);
+  double acc = 0.0;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += kRB) acc += x[i] * y[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = s[0];
+}
+__global__ __launch_bounds__(kRB) void dot_stage2(const double* __restrict__ part, int nb,
+                                                  double* __restrict__ out) {
+  __shared__ double s[kRB];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kRB) acc += part[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = kRB / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = s[0];
+}
+
+unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kRB - 1) / kRB); }
+
+}  // namespace
+
+DeviceRegression::DeviceRegression(DeviceMultiEval& op) : op_(op) {
+  N_ = op.num_grid();
+  M_ = op.num_data();
+  if (N_ == 0) throw device_error("solver: grid not set");
+  const size_t nvec = 7;
+  check_hip(hipMalloc(&vec_, nvec * N_ * sizeof(double)), "hipMalloc(solver vectors)");
+  check_hip(hipMalloc(&tempM_, M_ * sizeof(double)), "hipMalloc(solver temp)");
+  check_hip(hipMalloc(&dotpart_, (kDotBlocks + 1) * sizeof(double)), "hipMalloc(dot)");
+  check_hip(hipHostMalloc(&hostscal_, sizeof(double)), "hipHostMalloc");
+  alpha_ = static_cast<double*>(vec_);
+  b_ = alpha_ + N_;
+  r_ = b_ + N_;
+  d_ = r_ + N_;
+  q_ = d_ + N_;
+  c_ = q_ + N_;
+  w_ = c_ + N_;
+}
+
+DeviceRegression::~DeviceRegression() {
+  (void)hipStreamSynchronize(op_.stream());
+  if (vec_) (void)hipFree(vec_);
+  if (tempM_) (void)hipFree(tempM_);
+  if (dotpart_) (void)hipFree(dotpart_);
+  if (hostscal_) (void)hipHostFree(hostscal_);
+}
+
+void DeviceRegression::apply(const double* v, double* q, double lambda, Precision p) {
+  op_.mult_device(v, tempM_, p);
+  op_.mult_transpose_device(tempM_, q, p);
+  hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(),
+                     static_cast<double>(M_) * lambda, v, q, static_cast<int64_t>(N_));
+  check_hip(hipGetLastError(), "axpy");
+  if (p == Precision::FP64)
+    ++fp64_matvecs_;
+  else
+    ++fp32_matvecs_;
+}
+
+double DeviceRegression::dot(const double* x, const double* y) {
+  const int64_t n = static_cast<int64_t>(N_);
+  const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
+  double* part = static_cast<double*>(dotpart_);
+  hipLaunchKernelGGL(dot_stage1, dim3(kDotBlocks), dim3(kRB), 0, op_.stream(), x, y, n, chunk,
+                     part);
+  hipLaunchKernelGGL(dot_stage2, dim3(1), dim3(kRB), 0, op_.stream(), part, kDotBlocks,
+                     part + kDotBlocks);
+  check_hip(hipGetLastError(), "dot");
+  check_hip(hipMemcpyAsync(hostscal_, part + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost,
+                           op_.stream()),
+            "dot copy");
+  check_hip(hipStreamSynchronize(op_.stream()), "dot sync");
+  return *static_cast<double*>(hostscal_);
+}
+
+void DeviceRegression::axpy(double a, const double* x, double* y) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), a, x, y,
+                     static_cast<int64_t>(N_));
+  check_hip(hipGetLastError(), "axpy");
+}
+void DeviceRegression::xpby(const double* x, double b, double* y) {
+  hipLaunchKernelGGL(xpby_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), x, b, y,
+                     static_cast<int64_t>(N_));
+  check_hip(hipGetLastError(), "xpby");
+}
+void DeviceRegression::sub(const double* x, const double* y, double* z) {
+  hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), x, y, z,
+                     static_cast<int64_t>(N_));
+  check_hip(hipGetLastError(), "sub");
+}
+void DeviceRegression::fill(double* y, double v) {
+  hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), y, v,
+                     static_cast<int64_t>(N_));
+  check_hip(hipGetLastError(), "fill");
+}
+void DeviceRegression::copy(const double* x, double* y) {
+  check_hip(hipMemcpyAsync(y, x, N_ * sizeof(double), hipMemcpyDeviceToDevice, op_.stream()),
+            "copy");
+}
+
+// ConjugateGradients::solve on device vectors; solves A x = rhs in place.
+// Returns the iteration count; delta values as in the reference.
+size_t DeviceRegression::cg(double* x, const double* rhs, double lambda, Precision p,
+                            size_t max_it, double eps, bool reuse, double* final_delta,
+                            double* start_delta) {
+  const double eps2 = eps * eps;
+  double delta0 = 0.0;
+  copy(rhs, r_);
+  if (reuse) {
+    fill(q_, 0.0);
+    apply(q_, w_, lambda, p);
+    sub(r_, w_, r_);
+    delta0 = dot(r_, r_) * eps2;
+  } else {
+    fill(x, 0.0);
+  }
+  apply(x, w_, lambda, p);
+  sub(r_, w_, r_);
+  copy(r_, d_);
+  double delta_new = dot(r_, r_);
+  if (!reuse) delta0 = delta_new * eps2;
+  if (start_delta) *start_delta = reuse ? delta0 / eps2 : delta_new;
+  size_t it = 0;
+  while (it < max_it && delta_new > delta0) {
+    apply(d_, q_, lambda, p);
+    const double dq = dot(d_, q_);
+    if (dq == 0.0) break;
+    const double a = delta_new / dq;
+    axpy(a, d_, x);
+    if ((it % 50) == 0 && it > 0) {
+      apply(x, w_, lambda, p);
+      sub(rhs, w_, r_);
+    } else {
+      axpy(-a, q_, r_);
+    }
+    const double delta_old = delta_new;
+    delta_new = dot(r_, r_);
+    const double beta = delta_new / delta_old;
+    xpby(r_, beta, d_);
+    ++it;
+  }
+  if (final_delta) *final_delta = delta_new;
+  return it;
+}
+
+SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
+                                    const SolveOptions& o) {
+  const auto t0 = std::chrono::steady_clock::now();
+  fp64_matvecs_ = fp32_matvecs_ = 0;
+  SolveResult res;
+  // b = B^T y in FP64 (SystemMatrixLeastSquaresIdentity::generateb)
+  check_hip(hipMemcpyAsync(tempM_, y_host, M_ * sizeof(double), hipMemcpyHostToDevice,
+                           op_.stream()),
+            "copy y");
+  op_.mult_transpose_device(tempM_, b_, Precision::FP64);
+  if (o.reuse)
+    check_hip(hipMemcpyAsync(alpha_, alpha_host, N_ * sizeof(double), hipMemcpyHostToDevice,
+                             op_.stream()),
+              "copy alpha");
+  const double bnorm = std::sqrt(dot(b_, b_));
+  if (!o.mixed) {
+    double fd = 0.0, sd = 0.0;
+    res.iterations = cg(alpha_, b_, o.lambda, Precision::FP64, o.max_iterations, o.epsilon,
+                        o.reuse, &fd, &sd);
+  } else {
+    // r = b - A64 alpha (alpha = 0 or reused)
+    if (!o.reuse) fill(alpha_, 0.0);
+    // the outer residual lives in c_, which cg() does not touch
+    apply(alpha_, q_, o.lambda, Precision::FP64);
+    sub(b_, q_, c_);
+    double rnorm = std::sqrt(dot(c_, c_));
+    double* corr = nullptr;
+    check_hip(hipMalloc(&corr, N_ * sizeof(double)), "hipMalloc(correction)");
+    double* rhs = nullptr;
+    check_hip(hipMalloc(&rhs, N_ * sizeof(double)), "hipMalloc(rhs)");
+    try {
+      size_t steps = 0;
+      while (rnorm > o.epsilon * bnorm && steps < o.max_refinements) {
+        copy(c_, rhs);
+        res.iterations += cg(corr, rhs, o.lambda, Precision::FP32, o.max_iterations,
+                             o.inner_epsilon, false, nullptr, nullptr);
+        axpy(1.0, corr, alpha_);
+        apply(alpha_, q_, o.lambda, Precision::FP64);
+        sub(b_, q_, c_);
+        rnorm = std::sqrt(dot(c_, c_));
+        ++steps;
+      }
+      res.refinements = steps;
+    } catch (...) {
+      (void)hipFree(corr);
+      (void)hipFree(rhs);
+      throw;
+    }
+    check_hip(hipFree(corr), "hipFree");
+    check_hip(hipFree(rhs), "hipFree");
+  }
+  // final FP64 residual ||b - A alpha|| / ||b||
+  apply(alpha_, q_, o.lambda, Precision::FP64);
+  sub(b_, q_, c_);
+  const double rn = std::sqrt(dot(c_, c_));
+  res.residual_rel = bnorm > 0 ? rn / bnorm : rn;
+  --fp64_matvecs_;  // the reporting residual is not part of the solve
+  check_hip(hipMemcpyAsync(alpha_host, alpha_, N_ * sizeof(double), hipMemcpyDeviceToHost,
+                           op_.stream()),
+            "copy alpha out");
+  check_hip(hipStreamSynchronize(op_.stream()), "sync");
+  res.fp64_matvecs = fp64_matvecs_;
+  res.fp32_matvecs = fp32_matvecs_;
+  res.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+void DeviceRegression::system_mult_host(const double* v, double* q, double lambda, Precision p) {
+  check_hip(hipMemcpyAsync(d_, v, N_ * sizeof(double), hipMemcpyHostToDevice, op_.stream()),
+            "copy v");
+  apply(d_, q_, lambda, p);
+  check_hip(hipMemcpyAsync(q, q_, N_ * sizeof(double), hipMemcpyDeviceToHost, op_.stream()),
+            "copy q");
+  check_hip(hipStreamSynchronize(op_.stream()), "sync");
+}
+
+}  // namespace sgpp_hip
