@@ -160,6 +160,8 @@ def ref():
             R.ref_grid_new.restype = _vp
             R.ref_grid_free.argtypes = [_vp]
             R.ref_grid_regular.argtypes = [_vp, _sz]
+            R.ref_grid_from_arrays.argtypes = [ctypes.c_int, _sz, _sz, _up, _up]
+            R.ref_grid_from_arrays.restype = _vp
             R.ref_grid_size.argtypes = [_vp]
             R.ref_grid_size.restype = _sz
             R.ref_grid_level_index.argtypes = [_vp, _up, _up, ctypes.POINTER(ctypes.c_uint8)]
@@ -188,12 +190,22 @@ def _rcheck(rc):
 class RefGrid:
     """The reference's own Grid (Linear / ModLinear)."""
 
-    def __init__(self, basis, dim):
+    def __init__(self, basis, dim, _handle=None):
         R = ref()
         if R is None:
             raise RuntimeError("reference build not available")
         self.basis, self.dim = basis, dim
-        self.h = R.ref_grid_new(basis, dim)
+        self.h = _handle if _handle is not None else R.ref_grid_new(basis, dim)
+
+    @classmethod
+    def from_arrays(cls, basis, level, index):
+        """Reference grid holding exactly these points in this order."""
+        lv, ix = _u32(level), _u32(index)
+        n, dim = lv.shape
+        R = ref()
+        if R is None:
+            raise RuntimeError("reference build not available")
+        return cls(basis, dim, _handle=R.ref_grid_from_arrays(basis, dim, n, _u(lv), _u(ix)))
 
     def __del__(self):
         if getattr(self, "h", None) and ref() is not None:

@@ -91,6 +91,20 @@ void* ref_grid_new(int basis, size_t dim) {
   return g;
 }
 void ref_grid_free(void* g) { delete static_cast<RefGrid*>(g); }
+// a grid holding exactly the given points, inserted in order with the
+// reference's HashGridStorage::insert (for operators that only read level/index)
+void* ref_grid_from_arrays(int basis, size_t dim, size_t n, const uint32_t* lv,
+                           const uint32_t* ix) {
+  RefGrid* g = new RefGrid();
+  g->grid.reset(basis == 0 ? Grid::createLinearGrid(dim) : Grid::createModLinearGrid(dim));
+  auto& st = g->grid->getStorage();
+  sgpp::base::HashGridPoint p(dim);
+  for (size_t s = 0; s < n; ++s) {
+    for (size_t k = 0; k < dim; ++k) p.set(k, lv[s * dim + k], ix[s * dim + k]);
+    st.insert(p);
+  }
+  return g;
+}
 int ref_grid_regular(void* g, size_t level) {
   return guard([&] { static_cast<RefGrid*>(g)->grid->getGenerator().regular(level); });
 }
