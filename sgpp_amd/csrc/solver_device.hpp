@@ -45,7 +45,7 @@ class DeviceRegression {
   void fill(double* y, double v);
   void copy(const double* x, double* y);
   size_t cg(double* x, const double* rhs, double lambda, Precision p, size_t max_it, double eps,
-            bool reuse, double* final_delta, double* start_delta);
+            bool reuse, double* final_delta, double* start_delta, bool skip_zero_apply);
 
   DeviceMultiEval& op_;
   size_t N_ = 0, M_ = 0;

@@ -171,7 +171,7 @@ void DeviceRegression::copy(const double* x, double* y) {
 // Returns the iteration count; delta values as in the reference.
 size_t DeviceRegression::cg(double* x, const double* rhs, double lambda, Precision p,
                             size_t max_it, double eps, bool reuse, double* final_delta,
-                            double* start_delta) {
+                            double* start_delta, bool skip_zero_apply) {
   const double eps2 = eps * eps;
   double delta0 = 0.0;
   copy(rhs, r_);
@@ -183,8 +183,12 @@ size_t DeviceRegression::cg(double* x, const double* rhs, double lambda, Precisi
   } else {
     fill(x, 0.0);
   }
-  apply(x, w_, lambda, p);
-  sub(r_, w_, r_);
+  // The reference applies A to the zero start vector too (ConjugateGradients.cpp:62);
+  // the inner solves of the mixed mode skip that product, whose result is exactly 0.
+  if (reuse || !skip_zero_apply) {
+    apply(x, w_, lambda, p);
+    sub(r_, w_, r_);
+  }
   copy(r_, d_);
   double delta_new = dot(r_, r_);
   if (!reuse) delta0 = delta_new * eps2;
@@ -230,13 +234,17 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
   if (!o.mixed) {
     double fd = 0.0, sd = 0.0;
     res.iterations = cg(alpha_, b_, o.lambda, Precision::FP64, o.max_iterations, o.epsilon,
-                        o.reuse, &fd, &sd);
+                        o.reuse, &fd, &sd, false);
   } else {
     // r = b - A64 alpha (alpha = 0 or reused)
-    if (!o.reuse) fill(alpha_, 0.0);
     // the outer residual lives in c_, which cg() does not touch
-    apply(alpha_, q_, o.lambda, Precision::FP64);
-    sub(b_, q_, c_);
+    if (o.reuse) {
+      apply(alpha_, q_, o.lambda, Precision::FP64);
+      sub(b_, q_, c_);
+    } else {
+      fill(alpha_, 0.0);
+      copy(b_, c_);
+    }
     double rnorm = std::sqrt(dot(c_, c_));
     double* corr = nullptr;
     check_hip(hipMalloc(&corr, N_ * sizeof(double)), "hipMalloc(correction)");
@@ -247,7 +255,7 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
       while (rnorm > o.epsilon * bnorm && steps < o.max_refinements) {
         copy(c_, rhs);
         res.iterations += cg(corr, rhs, o.lambda, Precision::FP32, o.max_iterations,
-                             o.inner_epsilon, false, nullptr, nullptr);
+                             o.inner_epsilon, false, nullptr, nullptr, true);
         axpy(1.0, corr, alpha_);
         apply(alpha_, q_, o.lambda, Precision::FP64);
         sub(b_, q_, c_);
