@@ -182,6 +182,70 @@ sgpp_hip_status sgpp_hip_regression_solve(sgpp_hip_meval* h, const double* y,
 sgpp_hip_status sgpp_hip_system_matrix_mult(sgpp_hip_meval* h, double lambda, const double* v,
                                             double* q, int precision);
 
+/* --------------------------------------------------------- served loop
+ * Resident models + continuous batching. The reference has no serving loop:
+ * its prediction path builds one operator per request
+ * (LearnerLeastSquaresIdentity::predict, datadriven/src/sgpp/datadriven/
+ * application/LearnerLeastSquaresIdentity.cpp:61-68); these calls replace that
+ * pattern for callers that evaluate many models.
+ */
+typedef struct sgpp_hip_server sgpp_hip_server;
+
+typedef struct sgpp_hip_server_config {
+  int device;
+  int precision;                /* SGPP_HIP_PREC_* of the served products */
+  size_t memory_budget;         /* bytes for models + batch buffers; 0 = 64 GiB */
+  size_t max_batch_queries;     /* queries per batch; 0 = 4 Mi */
+  size_t max_items;             /* work items per batch; 0 = 256 Ki */
+  size_t max_partial_doubles;   /* partial-sum capacity per batch; 0 = 128 Mi */
+  int max_inflight;             /* batches on the GPU at once; 0 = 2 */
+  int max_requests_per_batch;   /* 1 = one launch per request; 0 = 65536 */
+  int blocks;                   /* persistent grid size; 0 = 8 per CU */
+  int use_graph;                /* replay the batch kernels from a hipGraph */
+  double item_work;             /* lane-operations per work item; 0 = 2^25 */
+} sgpp_hip_server_config;
+
+enum {
+  SGPP_HIP_REQ_QUEUED = 0,
+  SGPP_HIP_REQ_RUNNING = 1,
+  SGPP_HIP_REQ_DONE = 2,
+  SGPP_HIP_REQ_FAILED_BUDGET = 3,
+  SGPP_HIP_REQ_FAILED_TIMEOUT = 4,
+  SGPP_HIP_REQ_CANCELLED = 5,
+  SGPP_HIP_REQ_FAILED_DEVICE = 6
+};
+
+typedef struct sgpp_hip_server_statistics {
+  uint64_t requests_done;
+  uint64_t requests_failed;
+  uint64_t queries_done;
+  uint64_t batches;
+  uint64_t kernel_launches;
+  double busy_seconds;          /* summed device time of the batches */
+} sgpp_hip_server_statistics;
+
+sgpp_hip_status sgpp_hip_server_create(const sgpp_hip_server_config* cfg, sgpp_hip_server** out);
+sgpp_hip_status sgpp_hip_server_destroy(sgpp_hip_server* s);
+/* level/index N x d, alpha N; SGPP_HIP_ERR_BUDGET when it does not fit */
+sgpp_hip_status sgpp_hip_server_admit_model(sgpp_hip_server* s, int basis, size_t dim, size_t N,
+                                            const uint32_t* level, const uint32_t* index,
+                                            const double* alpha, int* model_id);
+sgpp_hip_status sgpp_hip_server_evict_model(sgpp_hip_server* s, int model_id);
+sgpp_hip_status sgpp_hip_server_memory_used(sgpp_hip_server* s, size_t* bytes);
+/* queries: q x d row-major in [0,1]^d; deadline_s <= 0 means none */
+sgpp_hip_status sgpp_hip_server_submit(sgpp_hip_server* s, int model_id, size_t q,
+                                       const double* queries, double deadline_s,
+                                       uint64_t* request_id);
+/* *status receives SGPP_HIP_REQ_*; a finished request is released (its q
+ * results copied to out when DONE). wait_s < 0 blocks, 0 polls. */
+sgpp_hip_status sgpp_hip_server_poll(sgpp_hip_server* s, uint64_t request_id, double* out,
+                                     double wait_s, int* status, double* latency_s);
+sgpp_hip_status sgpp_hip_server_cancel(sgpp_hip_server* s, uint64_t request_id);
+/* host-forced fault: the next batch reports a device error on its requests */
+sgpp_hip_status sgpp_hip_server_inject_error(sgpp_hip_server* s);
+sgpp_hip_status sgpp_hip_server_drain(sgpp_hip_server* s);
+sgpp_hip_status sgpp_hip_server_stats(sgpp_hip_server* s, sgpp_hip_server_statistics* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,6 +125,20 @@ SIGNATURES = {
     "sgpp_hip_regression_solve": (_st, [_p, _dp, ctypes.POINTER(SolveParams), _dp,
                                         ctypes.POINTER(SolveInfo)]),
     "sgpp_hip_system_matrix_mult": (_st, [_p, ctypes.c_double, _dp, _dp, ctypes.c_int]),
+    "sgpp_hip_server_create": (_st, [ctypes.c_void_p, ctypes.POINTER(_p)]),
+    "sgpp_hip_server_destroy": (_st, [_p]),
+    "sgpp_hip_server_admit_model": (_st, [_p, ctypes.c_int, _sz, _sz, _up, _up, _dp,
+                                          ctypes.POINTER(ctypes.c_int)]),
+    "sgpp_hip_server_evict_model": (_st, [_p, ctypes.c_int]),
+    "sgpp_hip_server_memory_used": (_st, [_p, ctypes.POINTER(_sz)]),
+    "sgpp_hip_server_submit": (_st, [_p, ctypes.c_int, _sz, _dp, ctypes.c_double,
+                                     ctypes.POINTER(ctypes.c_uint64)]),
+    "sgpp_hip_server_poll": (_st, [_p, ctypes.c_uint64, _dp, ctypes.c_double,
+                                   ctypes.POINTER(ctypes.c_int), _dp]),
+    "sgpp_hip_server_cancel": (_st, [_p, ctypes.c_uint64]),
+    "sgpp_hip_server_inject_error": (_st, [_p]),
+    "sgpp_hip_server_drain": (_st, [_p]),
+    "sgpp_hip_server_stats": (_st, [_p, ctypes.c_void_p]),
 }
 
 F = {}
@@ -345,3 +359,103 @@ class MultiEval:
         check(F["sgpp_hip_regression_solve"](self.h, _dptr(y), ctypes.byref(prm), _dptr(alpha),
                                              ctypes.byref(info)))
         return alpha, {k: getattr(info, k) for k, _ in SolveInfo._fields_}
+
+
+# ------------------------------------------------------------- served loop
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+REQ_FAILED_BUDGET, REQ_FAILED_TIMEOUT, REQ_CANCELLED, REQ_FAILED_DEVICE = 3, 4, 5, 6
+REQ_NAMES = {0: "QUEUED", 1: "RUNNING", 2: "DONE", 3: "FAILED_BUDGET", 4: "FAILED_TIMEOUT",
+             5: "CANCELLED", 6: "FAILED_DEVICE"}
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("precision", ctypes.c_int),
+                ("memory_budget", ctypes.c_size_t), ("max_batch_queries", ctypes.c_size_t),
+                ("max_items", ctypes.c_size_t), ("max_partial_doubles", ctypes.c_size_t),
+                ("max_inflight", ctypes.c_int), ("max_requests_per_batch", ctypes.c_int),
+                ("blocks", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("item_work", ctypes.c_double)]
+
+
+class ServerStatistics(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_uint64), ("requests_failed", ctypes.c_uint64),
+                ("queries_done", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("kernel_launches", ctypes.c_uint64), ("busy_seconds", ctypes.c_double)]
+
+
+class Server:
+    """Resident models + continuous batching (sgpp_hip_server_*)."""
+
+    def __init__(self, precision=PREC_FP32, memory_budget=0, max_batch_queries=0, max_items=0,
+                 max_partial_doubles=0, max_inflight=2, max_requests_per_batch=0, blocks=0,
+                 use_graph=True, item_work=0.0, device=0):
+        cfg = ServerConfig(device, precision, memory_budget, max_batch_queries, max_items,
+                           max_partial_doubles, max_inflight, max_requests_per_batch, blocks,
+                           int(use_graph), item_work)
+        h = _p()
+        check(F["sgpp_hip_server_create"](ctypes.cast(ctypes.byref(cfg), ctypes.c_void_p),
+                                          ctypes.byref(h)))
+        self.h = h
+        self.dims = {}
+
+    def close(self):
+        h = getattr(self, "h", None)
+        if h and F is not None:
+            F["sgpp_hip_server_destroy"](h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def admit_model(self, basis, level, index, alpha):
+        lv = np.ascontiguousarray(level, dtype=np.uint32)
+        ix = np.ascontiguousarray(index, dtype=np.uint32)
+        a = _f64(alpha)
+        mid = ctypes.c_int(-1)
+        check(F["sgpp_hip_server_admit_model"](self.h, basis, lv.shape[1], lv.shape[0],
+                                               lv.ctypes.data_as(_up), ix.ctypes.data_as(_up),
+                                               _dptr(a), ctypes.byref(mid)))
+        self.dims[mid.value] = lv.shape[1]
+        return mid.value
+
+    def evict_model(self, model_id):
+        check(F["sgpp_hip_server_evict_model"](self.h, model_id))
+        self.dims.pop(model_id, None)
+
+    def memory_used(self):
+        n = _sz(0)
+        check(F["sgpp_hip_server_memory_used"](self.h, ctypes.byref(n)))
+        return n.value
+
+    def submit(self, model_id, queries, deadline_s=0.0):
+        X = _f64(queries)
+        rid = ctypes.c_uint64(0)
+        check(F["sgpp_hip_server_submit"](self.h, model_id, X.shape[0], _dptr(X), deadline_s,
+                                          ctypes.byref(rid)))
+        return rid.value, X.shape[0]
+
+    def poll(self, rid, q, wait_s=-1.0):
+        """(status, results or None, latency seconds); a finished request is released."""
+        out = np.empty(q)
+        st = ctypes.c_int(-1)
+        lat = ctypes.c_double(0)
+        check(F["sgpp_hip_server_poll"](self.h, rid, _dptr(out), wait_s, ctypes.byref(st),
+                                        ctypes.byref(lat)))
+        if st.value == REQ_DONE:
+            return st.value, out, lat.value
+        return st.value, None, lat.value
+
+    def cancel(self, rid):
+        return F["sgpp_hip_server_cancel"](self.h, rid) == OK
+
+    def inject_error(self):
+        check(F["sgpp_hip_server_inject_error"](self.h))
+
+    def drain(self):
+        check(F["sgpp_hip_server_drain"](self.h))
+
+    def stats(self):
+        s = ServerStatistics()
+        check(F["sgpp_hip_server_stats"](self.h, ctypes.cast(ctypes.byref(s), ctypes.c_void_p)))
+        return {k: getattr(s, k) for k, _ in ServerStatistics._fields_}

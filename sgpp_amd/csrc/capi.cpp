@@ -90,6 +90,10 @@ sgpp::base::GridType basis_of(int b) {
 
 }  // namespace
 
+namespace sgpp_hip {
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace sgpp_hip
+
 extern "C" {
 
 const char* sgpp_hip_last_error(void) { return g_last_error.c_str(); }

@@ -1,0 +1,152 @@
+"""Served loop: results against the oracle, identical alone and in a mix,
+and per-request failure handling (budget, deadline, cancel, injected device
+error) that never stops the loop.
+
+Tolerances: FP32-with-FP64-accumulation server 1e-5 relative, FP64 server
+1e-12 relative (||got - ref||_2 / ||ref||_2 per request)."""
+import numpy as np
+import pytest
+
+import oracle
+from sgpp_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def small_models(capi):
+    specs = []
+    g = capi.Grid.linear(2).regular(5)
+    specs.append((capi.BASIS_LINEAR, g))
+    g = capi.Grid.modlinear(5).regular(3)
+    g.refine_surplus(np.linspace(1.0, 2.0, g.size), 20)
+    specs.append((capi.BASIS_MODLINEAR, g))
+    g = capi.Grid.linear(10).regular(3)
+    specs.append((capi.BASIS_LINEAR, g))
+    g = capi.Grid.modlinear(20).regular(2)
+    specs.append((capi.BASIS_MODLINEAR, g))
+    out = []
+    for i, (basis, g) in enumerate(specs):
+        lv, ix = g.level_index()
+        alpha = workloads.normal(workloads.stream(50 + i, 0), g.size)
+        out.append((basis, lv, ix, alpha))
+    return out
+
+
+def small_server(capi, precision, **kw):
+    kw.setdefault("max_batch_queries", 1 << 18)
+    kw.setdefault("max_items", 1 << 14)
+    kw.setdefault("max_partial_doubles", 1 << 22)
+    kw.setdefault("memory_budget", 4 << 30)
+    return capi.Server(precision=precision, **kw)
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (0, 1e-12)])
+def test_results_match_oracle(gpu, prec, tol):
+    capi = gpu
+    srv = small_server(capi, prec)
+    models = small_models(capi)
+    ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    reqs = []
+    for k, q in enumerate([1, 7, 300, 1025, 5000]):
+        for m, (b, lv, ix, a) in zip(ids, models):
+            X = workloads.uniform_points(60 + k, q, lv.shape[1])
+            if prec == 1:
+                X = X.astype(np.float32).astype(np.float64)
+            rid, n = srv.submit(m, X)
+            reqs.append((rid, n, X, b, lv, ix, a))
+    for rid, n, X, b, lv, ix, a in reqs:
+        st, r, lat = srv.poll(rid, n)
+        assert st == capi.REQ_DONE, capi.REQ_NAMES[st]
+        assert rel(r, oracle.mult(b, X, lv, ix, a)) <= tol
+        assert lat > 0
+    s = srv.stats()
+    assert s["requests_done"] == len(reqs) and s["requests_failed"] == 0
+    assert s["batches"] >= 1
+    srv.close()
+
+
+def test_alone_and_in_a_mix_are_identical(gpu):
+    capi = gpu
+    srv = small_server(capi, 1)
+    models = small_models(capi)
+    ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    probes = [(ids[j], workloads.uniform_points(70 + j, q, models[j][1].shape[1]))
+              for j, q in ((1, 2049), (2, 333), (3, 5))]
+    alone = []
+    for m, X in probes:
+        rid, n = srv.submit(m, X)
+        st, r, _ = srv.poll(rid, n)
+        assert st == capi.REQ_DONE
+        alone.append(r)
+    # the same requests interleaved with traffic for every model, several times
+    for rep in range(3):
+        rids = []
+        for k in range(12):
+            m = ids[k % len(ids)]
+            rids.append(srv.submit(m, workloads.uniform_points(80 + k + 13 * rep, 100 + 37 * k,
+                                                               models[k % len(ids)][1].shape[1])))
+            if k % 4 == 1:
+                rids.append(("probe", srv.submit(*probes[(k // 4) % 3]), (k // 4) % 3))
+        for item in rids:
+            if item[0] == "probe":
+                (rid, n), j = item[1], item[2]
+                st, r, _ = srv.poll(rid, n)
+                assert st == capi.REQ_DONE
+                assert np.array_equal(r, alone[j]), "result differs in a mix"
+            else:
+                rid, n = item
+                assert srv.poll(rid, n)[0] == capi.REQ_DONE
+    srv.close()
+
+
+def test_failures_are_per_request(gpu):
+    capi = gpu
+    srv = small_server(capi, 1, max_inflight=1, max_requests_per_batch=1,
+                       max_batch_queries=1 << 16, max_partial_doubles=1 << 20)
+    models = small_models(capi)
+    ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    d0 = models[0][1].shape[1]
+    # a model over the memory budget is refused
+    big = capi.Grid.linear(12).regular(5)
+    lv, ix = big.level_index()
+    tiny = small_server(capi, 1, memory_budget=1 << 27, max_batch_queries=1 << 12,
+                        max_items=1 << 10, max_partial_doubles=1 << 16)
+    with pytest.raises(capi.SGppHIPError) as e:
+        tiny.admit_model(capi.BASIS_LINEAR, np.repeat(lv, 64, axis=0), np.repeat(ix, 64, axis=0),
+                         np.ones(lv.shape[0] * 64))
+    assert e.value.status_name == "BUDGET"
+    tiny.close()
+    # a request larger than a batch can hold fails alone
+    rid, n = srv.submit(ids[0], workloads.uniform_points(90, 70000, d0))
+    assert srv.poll(rid, n)[0] == capi.REQ_FAILED_BUDGET
+    # a deadline that has passed before scheduling
+    heavy = srv.submit(ids[2], workloads.uniform_points(91, 60000, models[2][1].shape[1]))
+    late = srv.submit(ids[0], workloads.uniform_points(92, 10, d0), deadline_s=1e-7)
+    assert srv.poll(*late)[0] == capi.REQ_FAILED_TIMEOUT
+    # cancel a request queued behind the heavy one
+    queued = srv.submit(ids[0], workloads.uniform_points(93, 10, d0))
+    if srv.cancel(queued[0]):
+        assert srv.poll(*queued)[0] == capi.REQ_CANCELLED
+    else:
+        assert srv.poll(*queued)[0] == capi.REQ_DONE
+    assert srv.poll(*heavy)[0] == capi.REQ_DONE
+    # an injected device error fails one batch; the loop keeps serving
+    srv.inject_error()
+    bad = srv.submit(ids[1], workloads.uniform_points(94, 50, models[1][1].shape[1]))
+    assert srv.poll(*bad)[0] == capi.REQ_FAILED_DEVICE
+    good = srv.submit(ids[1], workloads.uniform_points(95, 50, models[1][1].shape[1]))
+    st, r, _ = srv.poll(*good)
+    assert st == capi.REQ_DONE and np.all(np.isfinite(r))
+    # eviction frees the model; unknown ids and out-of-cube points are rejected
+    used = srv.memory_used()
+    srv.evict_model(ids[3])
+    assert srv.memory_used() < used
+    with pytest.raises(capi.SGppHIPError):
+        srv.submit(ids[3], workloads.uniform_points(96, 5, 20))
+    with pytest.raises(capi.SGppHIPError):
+        srv.submit(ids[0], np.full((3, d0), 1.5))
+    srv.close()
