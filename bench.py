@@ -38,6 +38,12 @@ def parse():
     ap.add_argument("--no-check", action="store_true", help="skip the accuracy sample")
     ap.add_argument("--cpu-seconds", type=float, default=4.0,
                     help="target seconds per CPU mult+multTranspose run")
+    ap.add_argument("--serve", action="store_true",
+                    help="run the served-loop workload (64 models, Poisson requests) instead")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--solve", action="store_true",
+                    help="run the regression-solve workload (FP64 CG vs mixed precision) instead")
+    ap.add_argument("--inner-epsilon", type=float, default=1e-5)
     ap.add_argument("--grid-target", type=int, default=100000)
     ap.add_argument("--data-points", type=int, default=1000000)
     return ap.parse_args()
@@ -126,12 +132,191 @@ def cpu_baseline(basis, X, grid, lv, ix, alpha, y, seconds):
     }
 
 
+def drive_server(srv, ids, c, warm=32):
+    """Replay the request stream in real time against one server; returns
+    per-request latency (arrival -> result), completion times and outputs."""
+    import queue
+    import threading
+
+    import numpy as np
+    dims = [m["dim"] for m in c["models"]]
+    for k in range(warm):  # warm-up outside the timed trace
+        m = int(c["model"][k])
+        X = c["pools"][dims[m]][:64]
+        rid, q = srv.submit(ids[m], X)
+        srv.poll(rid, q)
+    n = len(c["arrival"])
+    lat = np.full(n, np.nan)
+    status = np.zeros(n, dtype=np.int64)
+    outs = {}
+    keep = set(range(0, n, max(1, n // 64)))
+    pend = queue.Queue()
+    t0 = time.perf_counter() + 0.05
+
+    def submitter():
+        for k in range(n):
+            ta = t0 + float(c["arrival"][k])
+            now = time.perf_counter()
+            if ta > now:
+                time.sleep(ta - now)
+            m, q, off = int(c["model"][k]), int(c["q"][k]), int(c["offset"][k])
+            X = c["pools"][dims[m]][off:off + q]
+            lag = time.perf_counter() - ta
+            rid, _ = srv.submit(ids[m], X)
+            pend.put((k, rid, q, lag))
+        pend.put(None)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    while True:
+        item = pend.get()
+        if item is None:
+            break
+        k, rid, q, lag = item
+        st, out, l = srv.poll(rid, q, wait_s=-1.0)
+        status[k] = st
+        lat[k] = lag + l
+        if k in keep and out is not None:
+            outs[k] = out
+    th.join()
+    done = status == 2
+    t_complete = c["arrival"] + lat
+    span = float(np.nanmax(t_complete[done]) - c["arrival"][0])
+    evals = float(c["q"][done].sum())
+    return dict(evals_per_s=evals / span, requests_done=int(done.sum()), requests=n,
+                p50_ms=float(np.percentile(lat[done], 50) * 1e3),
+                p99_ms=float(np.percentile(lat[done], 99) * 1e3),
+                mean_ms=float(np.mean(lat[done]) * 1e3), span_s=span,
+                stats=srv.stats()), outs
+
+
+def serve_bench(args):
+    import numpy as np
+
+    from sgpp_amd import capi, workloads
+    t = time.perf_counter()
+    c = workloads.config_serving(duration=args.serve_seconds)
+    gen_s = time.perf_counter() - t
+    N = np.array([m["level"].shape[0] for m in c["models"]])
+    D = np.array([m["dim"] for m in c["models"]])
+    flops = float(sum(6 * D[m] * N[m] * q for m, q in zip(c["model"], c["q"])))
+    runs = {}
+    outputs = {}
+    for name, kw in (("served", dict(precision=capi.PREC_FP32, max_inflight=2)),
+                     ("one_launch_per_request_fp64",
+                      dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
+                           use_graph=False))):
+        srv = capi.Server(**kw)
+        ids = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
+               for m in c["models"]]
+        res, outs = drive_server(srv, ids, c)
+        res["achieved_tflops_6d"] = flops * res["requests_done"] / len(c["q"]) / res["span_s"] / 1e12
+        runs[name] = res
+        outputs[name] = outs
+        srv.close()
+    errs = [np.linalg.norm(outputs["served"][k] - outputs["one_launch_per_request_fp64"][k]) /
+            max(np.linalg.norm(outputs["one_launch_per_request_fp64"][k]), 1e-300)
+            for k in outputs["served"] if k in outputs["one_launch_per_request_fp64"]]
+    base = runs["one_launch_per_request_fp64"]
+    line = {
+        "metric": "served evals/s + p50/p99 latency",
+        "value": runs["served"]["evals_per_s"],
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "p50_ms": runs["served"]["p50_ms"],
+        "p99_ms": runs["served"]["p99_ms"],
+        "dtype": "fp32 products, fp64 accumulation",
+        "data": "synthetic, seeded: 64 models, Poisson %.0f req/s for %.1f s" % (c["rate"],
+                                                                              c["duration"]),
+        "config": {"workload": "served_64_models_poisson_%dreqs" % int(c["rate"]),
+                   "models": len(c["models"]), "grid_points_total": int(N.sum()),
+                   "requests": len(c["q"]), "mean_queries": float(c["q"].mean()),
+                   "offered_evals_per_s": float(c["q"].sum() / c["duration"]),
+                   "offered_tflops_6d": flops / c["duration"] / 1e12},
+        "served": runs["served"],
+        "baseline": base,
+        "evals_ratio_vs_baseline": runs["served"]["evals_per_s"] / base["evals_per_s"],
+        "p99_ratio_vs_baseline": runs["served"]["p99_ms"] / base["p99_ms"],
+        "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
+        "setup_s": gen_s,
+    }
+    print(json.dumps(line))
+
+
+def solve_bench(args):
+    """Regression solve (Linear d=8 level 6, 500,000 points, lambda 1e-4) to a
+    1e-10 relative residual: plain FP64 CG against FP32 products with FP64 CG
+    vectors and FP64 iterative refinement, alpha checked against the reference's
+    FP64 CPU solve (tests/golden/regression_alpha.npz) when present."""
+    import numpy as np
+    import torch
+
+    from sgpp_amd import capi, workloads
+    c = workloads.config_regression()
+    lv, ix = c["grid"].level_index()
+    X, y, lam = c["X"], c["y"], c["lam"]
+    M, N, d = X.shape[0], lv.shape[0], lv.shape[1]
+    op = capi.MultiEval(capi.BASIS_LINEAR, X, precision=capi.PREC_FP32, keep_fp64=True)
+    op.prepare(lv, ix)
+    dev = torch.device("cuda:0")
+    d_a = torch.tensor(workloads.normal(9, N), device=dev)
+    d_y = torch.tensor(y, device=dev)
+    d_r = torch.empty(M, dtype=torch.float64, device=dev)
+    d_b = torch.empty(N, dtype=torch.float64, device=dev)
+    kern = {}
+    for name, prec in (("fp64", capi.PREC_FP64), ("fp32", capi.PREC_FP32)):
+        step_s, mult_ms, multT_ms = timed_steps(op, prec, d_a, d_y, d_r, d_b, 5, 1, torch)
+        kern[name] = {"pair_evals_per_s": 2.0 * M * N / step_s, "mult_ms": mult_ms,
+                      "multT_ms": multT_ms}
+    ref = None
+    rpath = os.path.join(ROOT, "tests", "golden", "regression_alpha.npz")
+    if os.path.exists(rpath):
+        ref = np.load(rpath)
+    runs = {}
+    for name, kw in (("fp64_cg", dict(mode=capi.SOLVE_FP64)),
+                     ("mixed_refinement", dict(mode=capi.SOLVE_MIXED,
+                                               inner_epsilon=args.inner_epsilon,
+                                               max_refinements=40))):
+        alpha, info = op.solve(y, lam, max_iterations=10000, epsilon=1e-10, **kw)
+        pairs = 2.0 * M * N * (info["fp64_matvecs"] + info["fp32_matvecs"])
+        info["pair_evals_per_s"] = pairs / info["seconds"]
+        if ref is not None:
+            info["alpha_rel_err_vs_cpu_fp64"] = float(np.linalg.norm(alpha - ref["alpha"]) /
+                                                      np.linalg.norm(ref["alpha"]))
+        runs[name] = info
+    line = {
+        "metric": "regression solve time to a 1e-10 relative residual, FP32+FP64 refinement vs FP64",
+        "value": runs["fp64_cg"]["seconds"] / runs["mixed_refinement"]["seconds"],
+        "unit": "x speedup (FP64 CG time / mixed time)",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "config": {"workload": c["name"], "grid_points": N, "data_points": M, "dim": d,
+                   "lambda": lam, "epsilon": 1e-10, "inner_epsilon": args.inner_epsilon},
+        "kernels": kern,
+        "kernel_speedup_fp32_vs_fp64": kern["fp32"]["pair_evals_per_s"] / kern["fp64"]["pair_evals_per_s"],
+        "fp64_cg": runs["fp64_cg"],
+        "mixed_refinement": runs["mixed_refinement"],
+        "cpu_reference": ({"iterations": int(ref["iterations"]), "seconds": float(ref["seconds"]),
+                           "threads": int(ref["threads"]),
+                           "solver": "reference ConjugateGradients on OperationMultipleEvalSubspaceCombined"}
+                          if ref is not None else None),
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box of this build has one GPU, "
               "so --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        serve_bench(args)
+        return
+    if args.solve:
+        solve_bench(args)
+        return
     import numpy as np
     import torch
 

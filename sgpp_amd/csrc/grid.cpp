@@ -337,6 +337,17 @@ void GridGenerator::refine(RefinementFunctor& functor, std::vector<size_t>* adde
     }
   }
   const double thr = functor.getRefinementThreshold();
+  // children of a level-31 point would need level 32 and index 2^32 - 1 or
+  // more in some later step; the reference's 32-bit index silently wraps
+  // there, so the limit is checked before the grid is touched
+  for (const ref_pair& pr : collection) {
+    if (pr.second < thr) continue;
+    const HashGridPoint& p = st.getPoint(pr.first);
+    for (size_t d = 0; d < st.getDimension(); ++d)
+      if (p.getLevel(d) >= kMaxRefineLevel)
+        throw generation_exception("refinement would create a point above level " +
+                                   std::to_string(kMaxRefineLevel));
+  }
   for (const ref_pair& pr : collection) {
     if (pr.second >= thr) {
       // HashRefinement::refineGridpoint (HashRefinement.cpp:228-236)

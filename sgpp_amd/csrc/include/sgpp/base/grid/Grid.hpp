@@ -206,6 +206,10 @@ class SurplusRefinementFunctor : public RefinementFunctor {
   double thr_;
 };
 
+// Highest level refinement may create: refine() throws generation_exception
+// instead of refining a point that already has this level in some dimension.
+constexpr level_t kMaxRefineLevel = 31;
+
 class GridGenerator {
  public:
   explicit GridGenerator(HashGridStorage& s) : storage_(s) {}

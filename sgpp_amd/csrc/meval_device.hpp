@@ -104,6 +104,18 @@ class DeviceMultiEval {
   void ensure(void** buf, size_t* cap, size_t bytes);
   void record_start();
   void record_stop();
+  template <typename T>
+  void run_mult(const T* X, const T* LI, const double* d_alpha, double* d_result,
+                hipEvent_t* tev);
+  template <typename T>
+  void run_multT(const T* X, const T* LI, const double* d_y, double* d_result, hipEvent_t* tev);
+
+  // ModLinear grids are stored sorted by each point's set of non-constant
+  // dimensions: perm_[stored position] = caller's index; tmask_[tile] = union
+  // of those sets over each 64-point tile
+  bool skip_ = false;
+  void* perm_ = nullptr;   size_t perm_cap_ = 0;
+  void* tmask_ = nullptr;  size_t tmask_cap_ = 0;
 
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;

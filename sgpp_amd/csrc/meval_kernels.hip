@@ -9,15 +9,21 @@
 // work (no contraction), so there is no MFMA anywhere in this file.
 //
 // mult (r = B alpha): lanes own data points (K per lane, coordinates held in
-//   registers for the whole launch); grid tiles of TG points are staged in
+//   registers for the whole launch); grid tiles of 64 points are staged in
 //   LDS and read as wave-uniform broadcasts. Work is split over data tiles
 //   (blockIdx.x) and G grid chunks (blockIdx.y); the G FP64 partial rows are
 //   summed in a fixed order by reduce_parts.
 // multTranspose (a = B^T y): lanes own grid points (K per lane, L/I in
-//   registers); data tiles of TM points (coordinates + y) are staged in LDS
+//   registers); data tiles of 128 points (coordinates + y) are staged in LDS
 //   and broadcast. Work is split over grid tiles (blockIdx.x) and S data
 //   segments (blockIdx.y); the S FP64 partial rows are summed in a fixed
 //   order. No atomics are used anywhere: results are deterministic.
+// Constant factors: a ModLinear level-1 factor is exactly 1. For ModLinear
+//   grids the points are kept sorted by their set of non-constant dimensions
+//   (an internal permutation, undone on input and output), every 64-point
+//   tile carries the union of its points' sets, and mult skips the other
+//   dimensions with a wave-uniform scalar branch. Skipping a factor of
+//   exactly 1 leaves every result bit-identical.
 // Precision: FP64 throughout, or FP32 basis products and per-tile sums with
 //   the per-tile partials accumulated in FP64 registers and FP64 partials.
 //
@@ -32,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <numeric>
 #include <sstream>
 
 #include "meval_device.hpp"
@@ -41,7 +48,7 @@ namespace sgpp_hip {
 namespace {
 
 constexpr int kBlock = 256;  // 4 waves of 64
-constexpr int kTG = 64;      // grid points per LDS tile (mult)
+constexpr int kTG = 64;      // grid points per LDS tile (mult) and per mask tile
 constexpr int kTM = 128;     // data points per LDS tile (multTranspose)
 constexpr int kMaxTemplD = 20;
 
@@ -77,12 +84,14 @@ constexpr int multT_k(int D) {
 
 // ---------------------------------------------------------------- mult
 // X: [D][ldx] SoA, LI: [Npad][D][2], A: [Npad] (alpha*scale, zero padded),
+// tmask: [Npad/64] non-constant dimensions of each 64-point tile (kSkip only),
 // part: [G][ldx]. Block covers kBlock*K data points and grid chunk
-// [blockIdx.y*chunk, min(+chunk, ntiles*kTG)).
-template <int D, int K, typename T>
+// [blockIdx.y*chunk, min(+chunk, jend_all)).
+template <int D, int K, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
                                                       const T* __restrict__ A,
+                                                      const uint32_t* __restrict__ tmask,
                                                       double* __restrict__ part, int chunk,
                                                       int jend_all) {
   __shared__ T sLI[kTG * D * 2];
@@ -101,6 +110,7 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
   const int jb = blockIdx.y * chunk;
   const int je = min(jb + chunk, jend_all);
   for (int j0 = jb; j0 < je; j0 += kTG) {
+    const uint32_t mask = kSkip ? tmask[j0 / kTG] : 0xffffffffu;
     __syncthreads();
     for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
       sLI[e] = LI[static_cast<int64_t>(j0) * D * 2 + e];
@@ -110,14 +120,23 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
     for (int j = 0; j < kTG; ++j) {
-      T p[K];
+      // all (L, I) of the point first: one batch of LDS reads, one wait
+      T L[D], I[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T L = sLI[(j * D + d) * 2];
-        const T I = sLI[(j * D + d) * 2 + 1];
+        L[d] = sLI[(j * D + d) * 2];
+        I[d] = sLI[(j * D + d) * 2 + 1];
+      }
+      T p[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) p[k] = T(1);
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (kSkip && !((mask >> d) & 1u)) continue;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          p[k] = (d == 0) ? hat_first(L, x[k][d], I) : hat_step(p[k], L, x[k][d], I);
+          p[k] = (!kSkip && d == 0) ? hat_first(L[d], x[k][d], I[d])
+                                    : hat_step(p[k], L[d], x[k][d], I[d]);
       }
       const T a = sA[j];
 #pragma unroll
@@ -159,17 +178,27 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // ------------------------------------------------------- multTranspose
 // X: [D][ldx] SoA, Y: [ldx] (zero padded), LI: [Npad][D][2],
 // part: [S][ldp]. Block covers grid points [blockIdx.x*kBlock*K, +kBlock*K)
-// and data segment [blockIdx.y*seg, min(+seg, mend)).
-template <int D, int K, typename T>
+// and data segment [blockIdx.y*seg, min(+seg, mend)). With kSkip each wave
+// evaluates only the union of the non-constant dimensions of its K tiles.
+template <int D, int K, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
+                                                       const uint32_t* __restrict__ tmask,
                                                        double* __restrict__ part, int64_t ldp,
                                                        int64_t seg, int64_t mend) {
   constexpr int kVec = 16 / static_cast<int>(sizeof(T));
   constexpr int RS = ((D + 1 + kVec - 1) / kVec) * kVec;  // LDS row stride
   __shared__ T sX[kTM * RS];
-  const int64_t j0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + threadIdx.x;
+  const int64_t jblock = static_cast<int64_t>(blockIdx.x) * (kBlock * K);
+  const int64_t j0 = jblock + threadIdx.x;
+  uint32_t mask = 0xffffffffu;
+  if (kSkip) {
+    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
+    mask = 0u;
+#pragma unroll
+    for (int k = 0; k < K; ++k) mask |= tmask[(jblock + wave * 64 + k * kBlock) / kTG];
+  }
   T L[K][D], I[K][D];
 #pragma unroll
   for (int k = 0; k < K; ++k)
@@ -197,15 +226,22 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
     for (int k = 0; k < K; ++k) acc[k] = T(0);
     for (int m = 0; m < kTM; ++m) {
       const T* row = sX + m * RS;
+      // the whole row first: one batch of LDS reads, one wait
+      T xr[D + 1];
+#pragma unroll
+      for (int d = 0; d <= D; ++d) xr[d] = row[d];
       T p[K];
 #pragma unroll
+      for (int k = 0; k < K; ++k) p[k] = T(1);
+#pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T xv = row[d];
+        if (kSkip && !((mask >> d) & 1u)) continue;
 #pragma unroll
         for (int k = 0; k < K; ++k)
-          p[k] = (d == 0) ? hat_first(L[k][d], xv, I[k][d]) : hat_step(p[k], L[k][d], xv, I[k][d]);
+          p[k] = (!kSkip && d == 0) ? hat_first(L[k][d], xr[d], I[k][d])
+                                    : hat_step(p[k], L[k][d], xr[d], I[k][d]);
       }
-      const T y = row[D];
+      const T y = xr[D];
 #pragma unroll
       for (int k = 0; k < K; ++k) acc[k] = fmaT(y, p[k], acc[k]);
     }
@@ -242,11 +278,19 @@ __global__ __launch_bounds__(kBlock) void multT_kernel_generic(const T* __restri
 }
 
 // ------------------------------------------------------------- helpers
+// A[s] = alpha[perm[s]] * scale[s] for stored position s (perm may be null)
 template <typename T>
 __global__ void prep_alpha_kernel(const double* __restrict__ alpha, const double* __restrict__ scale,
-                                  T* __restrict__ A, int64_t n, int64_t npad) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j < npad) A[j] = (j < n) ? static_cast<T>(alpha[j] * scale[j]) : T(0);
+                                  const int32_t* __restrict__ perm, T* __restrict__ A, int64_t n,
+                                  int64_t npad) {
+  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= npad) return;
+  if (s < n) {
+    const int64_t j = perm ? perm[s] : s;
+    A[s] = static_cast<T>(alpha[j] * scale[s]);
+  } else {
+    A[s] = T(0);
+  }
 }
 
 template <typename T>
@@ -256,43 +300,55 @@ __global__ void prep_y_kernel(const double* __restrict__ y, T* __restrict__ Y, i
   if (m < npad) Y[m] = (m < n) ? static_cast<T>(y[m]) : T(0);
 }
 
-// out[i] = (scale ? scale[i] : 1) * sum_{p < nparts} part[p*ld + i], fixed order.
+// out[perm ? perm[i] : i] = (scale ? scale[i] : 1) * sum_{p < nparts} part[p*ld + i],
+// summed in a fixed order
 __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld, int nparts,
-                                    const double* __restrict__ scale, double* __restrict__ out,
+                                    const double* __restrict__ scale,
+                                    const int32_t* __restrict__ perm, double* __restrict__ out,
                                     int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
   for (int p = 0; p < nparts; ++p) s += part[p * ld + i];
-  out[i] = scale ? s * scale[i] : s;
+  out[perm ? perm[i] : i] = scale ? s * scale[i] : s;
 }
 
 // ------------------------------------------------------------ dispatch
 template <typename T, int D>
-void launch_mult_D(const T* X, int64_t ldx, const T* LI, const T* A, double* part, int chunk,
-                   int jend, int G, int64_t Mpad, hipStream_t s) {
+void launch_mult_D(bool skip, const T* X, int64_t ldx, const T* LI, const T* A,
+                   const uint32_t* tm, double* part, int chunk, int jend, int G, int64_t Mpad,
+                   hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
-  hipLaunchKernelGGL((mult_kernel<D, K, T>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, part, chunk,
-                     jend);
+  if (skip)
+    hipLaunchKernelGGL((mult_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, tm,
+                       part, chunk, jend);
+  else
+    hipLaunchKernelGGL((mult_kernel<D, K, T, false>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, tm,
+                       part, chunk, jend);
 }
 
 template <typename T, int D>
-void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, double* part, int64_t ldp,
-                    int64_t seg, int64_t mend, int S, int64_t N, hipStream_t s) {
+void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
+                    const uint32_t* tm, double* part, int64_t ldp, int64_t seg, int64_t mend,
+                    int S, int64_t N, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
   dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-  hipLaunchKernelGGL((multT_kernel<D, K, T>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI, part, ldp,
-                     seg, mend);
+  if (skip)
+    hipLaunchKernelGGL((multT_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI, tm,
+                       part, ldp, seg, mend);
+  else
+    hipLaunchKernelGGL((multT_kernel<D, K, T, false>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI,
+                       tm, part, ldp, seg, mend);
 }
 
 template <typename T>
-using MultFn = void (*)(const T*, int64_t, const T*, const T*, double*, int, int, int, int64_t,
-                        hipStream_t);
+using MultFn = void (*)(bool, const T*, int64_t, const T*, const T*, const uint32_t*, double*, int,
+                        int, int, int64_t, hipStream_t);
 template <typename T>
-using MultTFn = void (*)(const T*, int64_t, const T*, const T*, double*, int64_t, int64_t, int64_t,
-                         int, int64_t, hipStream_t);
+using MultTFn = void (*)(bool, const T*, int64_t, const T*, const T*, const uint32_t*, double*,
+                         int64_t, int64_t, int64_t, int, int64_t, hipStream_t);
 
 template <typename T, int... Ds>
 struct Table {
@@ -407,7 +463,7 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, scale_, A_, Y_, part_, hin_, hout_};
+  void* bufs[] = {X64_, X32_, LI64_, LI32_, scale_, perm_, tmask_, A_, Y_, part_, hin_, hout_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto* pool : {&tm_mult_, &tm_multT_})
@@ -435,17 +491,51 @@ void DeviceMultiEval::ensure(void** buf, size_t* cap, size_t bytes) {
 void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, size_t N) {
   if (N == 0) throw input_error("empty grid");
   if (N > (size_t(1) << 30)) throw input_error("grid too large");
+  // non-constant dimensions of each point; ModLinear level-1 factors are 1
+  std::vector<uint32_t> pmask(N);
+  const uint32_t full = D_ >= 32 ? 0xffffffffu : ((1u << D_) - 1u);
+  const bool skip = basis_ == Basis::ModLinear && D_ <= static_cast<size_t>(kMaxTemplD);
+  for (size_t j = 0; j < N; ++j) {
+    uint32_t m = 0;
+    for (size_t d = 0; d < D_; ++d)
+      if (!skip || level[j * D_ + d] > 1) m |= 1u << d;
+    pmask[j] = skip ? m : full;
+  }
+  // storage order: sorted by the set of non-constant dimensions (stable)
+  std::vector<int32_t> order(N);
+  std::iota(order.begin(), order.end(), 0);
+  if (skip)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return pmask[a] < pmask[b]; });
+  std::vector<uint32_t> lv(N * D_), ix(N * D_);
+  for (size_t s = 0; s < N; ++s)
+    for (size_t d = 0; d < D_; ++d) {
+      lv[s * D_ + d] = level[order[s] * D_ + d];
+      ix[s * D_ + d] = index[order[s] * D_ + d];
+    }
   std::vector<double> li, sc;
-  encode_grid(basis_, D_, N, level, index, li, sc);
-  N_ = N;
-  Npad_ = ((N_ + kGridPad - 1) / kGridPad) * kGridPad;
+  encode_grid(basis_, D_, N, lv.data(), ix.data(), li, sc);
+  const size_t Npad = ((N + kGridPad - 1) / kGridPad) * kGridPad;
   // padded points: (L, I) = (0, 0) -> factor 1, weight 0 through A / ignored rows
-  li.resize(Npad_ * D_ * 2, 0.0);
-  sc.resize(Npad_, 1.0);
+  li.resize(Npad * D_ * 2, 0.0);
+  sc.resize(Npad, 1.0);
+  std::vector<uint32_t> tm(Npad / kTG, 0u);
+  for (size_t s = 0; s < N; ++s) tm[s / kTG] |= pmask[order[s]];
   check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  N_ = N;
+  Npad_ = Npad;
+  skip_ = skip;
   ensure(reinterpret_cast<void**>(&scale_), &gridcap_scale_, Npad_ * sizeof(double));
   check_hip(hipMemcpy(scale_, sc.data(), Npad_ * sizeof(double), hipMemcpyHostToDevice),
             "upload scale");
+  ensure(&tmask_, &tmask_cap_, tm.size() * sizeof(uint32_t));
+  check_hip(hipMemcpy(tmask_, tm.data(), tm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+            "upload tile masks");
+  if (skip_) {
+    ensure(&perm_, &perm_cap_, N_ * sizeof(int32_t));
+    check_hip(hipMemcpy(perm_, order.data(), N_ * sizeof(int32_t), hipMemcpyHostToDevice),
+              "upload permutation");
+  }
   if (serves(Precision::FP64)) {
     ensure(&LI64_, &gridcap_li64_, li.size() * sizeof(double));
     check_hip(hipMemcpy(LI64_, li.data(), li.size() * sizeof(double), hipMemcpyHostToDevice),
@@ -462,24 +552,22 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
 void DeviceMultiEval::record_start() { check_hip(hipEventRecord(ev0_, stream_), "hipEventRecord"); }
 void DeviceMultiEval::record_stop() { check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord"); }
 
-namespace {
 template <typename T>
-void do_mult(const T* X, size_t Mpad, size_t D, size_t N, const T* LI, const double* scale,
-             const double* d_alpha, double* d_result, size_t M, void*& A, size_t& Acap,
-             void*& part, size_t& partcap, size_t Npad, const LaunchConfig& cfg, int cus,
-             hipStream_t s, DeviceMultiEval* self, int* G_out,
-             void (DeviceMultiEval::*ens)(void**, size_t*, size_t), hipEvent_t* tev) {
-  (self->*ens)(&A, &Acap, Npad * sizeof(T));
-  hipLaunchKernelGGL(prep_alpha_kernel<T>, dim3(ceil_div(Npad, 256)), dim3(256), 0, s, d_alpha,
-                     scale, static_cast<T*>(A), static_cast<int64_t>(N),
-                     static_cast<int64_t>(Npad));
+void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, double* d_result,
+                               hipEvent_t* tev) {
+  hipStream_t s = stream_;
+  const int32_t* perm = skip_ ? static_cast<const int32_t*>(perm_) : nullptr;
+  ensure(&A_, &A_cap_, Npad_ * sizeof(T));
+  hipLaunchKernelGGL(prep_alpha_kernel<T>, dim3(ceil_div(Npad_, 256)), dim3(256), 0, s, d_alpha,
+                     scale_, perm, static_cast<T*>(A_), static_cast<int64_t>(N_),
+                     static_cast<int64_t>(Npad_));
   check_hip(hipGetLastError(), "prep_alpha");
-  const int K = mult_k_rt<T>(static_cast<int>(D));
-  const int ntiles = ceil_div(N, kTG);
-  const int bx = static_cast<int>(Mpad / (kBlock * K));
-  int G = cfg.mult_grid_chunks;
+  const int K = mult_k_rt<T>(static_cast<int>(D_));
+  const int ntiles = ceil_div(N_, kTG);
+  const int bx = static_cast<int>(Mpad_ / (kBlock * K));
+  int G = cfg_.mult_grid_chunks;
   if (G <= 0) {
-    const int target = cfg.target_blocks > 0 ? cfg.target_blocks : 8 * cus;
+    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : 8 * num_cus_;
     G = std::max(1, ceil_div(target, bx));
     G = std::min(G, std::max(1, ntiles / 4));  // at least 4 tiles per chunk
   }
@@ -488,47 +576,45 @@ void do_mult(const T* X, size_t Mpad, size_t D, size_t N, const T* LI, const dou
   G = ceil_div(ntiles, tiles_per);
   const int chunk = tiles_per * kTG;
   const int jend = ntiles * kTG;
-  (self->*ens)(&part, &partcap, static_cast<size_t>(G) * Mpad * sizeof(double));
-  double* P = static_cast<double*>(part);
+  ensure(&part_, &part_cap_, static_cast<size_t>(G) * Mpad_ * sizeof(double));
+  double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
-  if (D <= static_cast<size_t>(kMaxTemplD)) {
-    MultFn<T> f = DimTable<T>::mult(static_cast<int>(D));
-    f(X, static_cast<int64_t>(Mpad), LI, static_cast<const T*>(A), P, chunk, jend, G,
-      static_cast<int64_t>(Mpad), s);
+  if (D_ <= static_cast<size_t>(kMaxTemplD)) {
+    MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_));
+    f(skip_, X, static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_),
+      static_cast<const uint32_t*>(tmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
-    dim3 grid(static_cast<unsigned>(Mpad / kBlock), static_cast<unsigned>(G));
+    dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     hipLaunchKernelGGL(mult_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
-                       static_cast<int64_t>(Mpad), LI, static_cast<const T*>(A), P, chunk, jend,
-                       static_cast<int>(D));
+                       static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_), P, chunk, jend,
+                       static_cast<int>(D_));
   }
   check_hip(hipGetLastError(), "mult kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
-  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M, 256)), dim3(256), 0, s, P,
-                     static_cast<int64_t>(Mpad), G, static_cast<const double*>(nullptr), d_result,
-                     static_cast<int64_t>(M));
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M_, 256)), dim3(256), 0, s, P,
+                     static_cast<int64_t>(Mpad_), G, static_cast<const double*>(nullptr),
+                     static_cast<const int32_t*>(nullptr), d_result, static_cast<int64_t>(M_));
   check_hip(hipGetLastError(), "reduce (mult)");
-  *G_out = G;
+  last_G_ = G;
 }
 
 template <typename T>
-void do_multT(const T* X, size_t Mpad, size_t D, size_t N, const T* LI, const double* scale,
-              const double* d_y, double* d_result, size_t M, void*& Y, size_t& Ycap, void*& part,
-              size_t& partcap, size_t Npad, const LaunchConfig& cfg, int cus, hipStream_t s,
-              DeviceMultiEval* self, int* S_out,
-              void (DeviceMultiEval::*ens)(void**, size_t*, size_t), hipEvent_t* tev) {
-  (self->*ens)(&Y, &Ycap, Mpad * sizeof(T));
-  hipLaunchKernelGGL(prep_y_kernel<T>, dim3(ceil_div(Mpad, 256)), dim3(256), 0, s, d_y,
-                     static_cast<T*>(Y), static_cast<int64_t>(M), static_cast<int64_t>(Mpad));
+void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, double* d_result,
+                                hipEvent_t* tev) {
+  hipStream_t s = stream_;
+  ensure(&Y_, &Y_cap_, Mpad_ * sizeof(T));
+  hipLaunchKernelGGL(prep_y_kernel<T>, dim3(ceil_div(Mpad_, 256)), dim3(256), 0, s, d_y,
+                     static_cast<T*>(Y_), static_cast<int64_t>(M_), static_cast<int64_t>(Mpad_));
   check_hip(hipGetLastError(), "prep_y");
-  const int K = multT_k_rt<T>(static_cast<int>(D));
+  const int K = multT_k_rt<T>(static_cast<int>(D_));
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
-  const int bx = ceil_div(N, cover);
-  // data end: only tiles that contain real points (rest of Mpad is padding)
-  const int64_t mtiles = ceil_div(M, kTM);
+  const int bx = ceil_div(N_, cover);
+  // data end: only tiles that contain real points (the rest of Mpad is padding)
+  const int64_t mtiles = ceil_div(M_, kTM);
   const int64_t mend = mtiles * kTM;
-  int S = cfg.multT_data_segs;
+  int S = cfg_.multT_data_segs;
   if (S <= 0) {
-    const int target = cfg.target_blocks > 0 ? cfg.target_blocks : 8 * cus;
+    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : 8 * num_cus_;
     S = std::max(1, ceil_div(target, bx));
     S = static_cast<int>(std::min<int64_t>(S, std::max<int64_t>(1, mtiles / 4)));
   }
@@ -536,27 +622,32 @@ void do_multT(const T* X, size_t Mpad, size_t D, size_t N, const T* LI, const do
   const int64_t tiles_per = (mtiles + S - 1) / S;
   S = static_cast<int>((mtiles + tiles_per - 1) / tiles_per);
   const int64_t seg = tiles_per * kTM;
-  (self->*ens)(&part, &partcap, static_cast<size_t>(S) * Npad * sizeof(double));
-  double* P = static_cast<double*>(part);
+  ensure(&part_, &part_cap_, static_cast<size_t>(S) * Npad_ * sizeof(double));
+  double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
-  if (D <= static_cast<size_t>(kMaxTemplD)) {
-    MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D));
-    f(X, static_cast<int64_t>(Mpad), static_cast<const T*>(Y), LI, P, static_cast<int64_t>(Npad),
-      seg, mend, S, static_cast<int64_t>(N), s);
+  if (D_ <= static_cast<size_t>(kMaxTemplD)) {
+    MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_));
+    // multTranspose evaluates every dimension: a wave's points span K tiles whose
+    // union of non-constant dimensions is close to all of them, and the
+    // per-dimension branches cost more than the factors they would skip
+    f(false, X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI,
+      static_cast<const uint32_t*>(tmask_), P, static_cast<int64_t>(Npad_), seg, mend, S,
+      static_cast<int64_t>(N_), s);
   } else {
-    dim3 grid(static_cast<unsigned>(ceil_div(N, kBlock)), static_cast<unsigned>(S));
+    dim3 grid(static_cast<unsigned>(ceil_div(N_, kBlock)), static_cast<unsigned>(S));
     hipLaunchKernelGGL(multT_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
-                       static_cast<int64_t>(Mpad), static_cast<const T*>(Y), LI, P,
-                       static_cast<int64_t>(Npad), seg, mend, static_cast<int>(D));
+                       static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, P,
+                       static_cast<int64_t>(Npad_), seg, mend, static_cast<int>(D_));
   }
   check_hip(hipGetLastError(), "multTranspose kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
-  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N, 256)), dim3(256), 0, s, P,
-                     static_cast<int64_t>(Npad), S, scale, d_result, static_cast<int64_t>(N));
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N_, 256)), dim3(256), 0, s, P,
+                     static_cast<int64_t>(Npad_), S, scale_,
+                     skip_ ? static_cast<const int32_t*>(perm_) : nullptr, d_result,
+                     static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "reduce (multTranspose)");
-  *S_out = S;
+  last_S_ = S;
 }
-}  // namespace
 
 void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Precision p) {
   if (N_ == 0) throw input_error("grid not set (call prepare)");
@@ -566,15 +657,11 @@ void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Preci
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
   if (p == Precision::FP64)
-    do_mult<double>(static_cast<const double*>(X64_), Mpad_, D_, N_,
-                    static_cast<const double*>(LI64_), scale_, d_alpha, d_result, M_, A_, A_cap_,
-                    part_, part_cap_, Npad_, cfg_, num_cus_, stream_, this, &last_G_,
-                    &DeviceMultiEval::ensure, tp ? tev : nullptr);
+    run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_), d_alpha,
+                     d_result, tp ? tev : nullptr);
   else
-    do_mult<float>(static_cast<const float*>(X32_), Mpad_, D_, N_,
-                   static_cast<const float*>(LI32_), scale_, d_alpha, d_result, M_, A_, A_cap_,
-                   part_, part_cap_, Npad_, cfg_, num_cus_, stream_, this, &last_G_,
-                   &DeviceMultiEval::ensure, tp ? tev : nullptr);
+    run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_), d_alpha,
+                    d_result, tp ? tev : nullptr);
   record_stop();
 }
 
@@ -586,15 +673,11 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
   if (p == Precision::FP64)
-    do_multT<double>(static_cast<const double*>(X64_), Mpad_, D_, N_,
-                     static_cast<const double*>(LI64_), scale_, d_y, d_result, M_, Y_, Y_cap_,
-                     part_, part_cap_, Npad_, cfg_, num_cus_, stream_, this, &last_S_,
-                     &DeviceMultiEval::ensure, tp ? tev : nullptr);
+    run_multT<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_), d_y,
+                      d_result, tp ? tev : nullptr);
   else
-    do_multT<float>(static_cast<const float*>(X32_), Mpad_, D_, N_,
-                    static_cast<const float*>(LI32_), scale_, d_y, d_result, M_, Y_, Y_cap_,
-                    part_, part_cap_, Npad_, cfg_, num_cus_, stream_, this, &last_S_,
-                    &DeviceMultiEval::ensure, tp ? tev : nullptr);
+    run_multT<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_), d_y,
+                     d_result, tp ? tev : nullptr);
   record_stop();
 }
 

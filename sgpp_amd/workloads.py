@@ -84,6 +84,15 @@ def cosine_product(c):
     return np.prod(1.0 + 0.5 * np.cos(np.pi * c), axis=1)
 
 
+def damped_cosine_product(c):
+    """f(x) = prod_j 4 x_j (1 - x_j) (1 + 0.5 cos(pi x_j)); zero on the
+    boundary of the cube. (Without the damping factor the boundary values
+    cannot be represented by a grid without boundary points, and surplus
+    refinement keeps halving the cells next to the faces until it runs out of
+    levels.)"""
+    return np.prod(4.0 * c * (1.0 - c) * (1.0 + 0.5 * np.cos(np.pi * c)), axis=1)
+
+
 def adaptive_grid(basis, dim, start_level, f, per_step, target, max_steps=10000):
     """Regular grid, then repeat {hierarchise f at the grid points; refine the
     per_step largest |surplus|} until the grid has >= target points.
@@ -150,12 +159,72 @@ def config_regression(M=500000, level=6, dim=8):
 
 
 def config_large_streaming(target=1000000, M=10000000):
-    """Linear d=20 refined against prod_j (1 + 0.5 cos(pi x_j)) to >= target
-    points; M points from an 8-component mixture (sigma 0.05, seed 6), stored
-    as FP32 values (the FP64 path evaluates the same values); lambda 1e-5."""
+    """Linear d=20: regular level 4, +200 refinements per step of the
+    hierarchised damped cosine product until >= target points; M points from
+    an 8-component mixture (sigma 0.05, means in [0.2,0.8]^20, clipped, seed 6),
+    stored as FP32 values (the FP64 path evaluates the same values);
+    y ~ N(0,1) (seed 6, tag 7); lambda 1e-5."""
     from . import capi
-    g, falpha, steps = adaptive_grid(capi.BASIS_LINEAR, 20, 4, cosine_product, 200, target)
+    g, falpha, steps = adaptive_grid(capi.BASIS_LINEAR, 20, 4, damped_cosine_product, 200,
+                                     target)
     X = gaussian_mixture(6, M, 20, 8, 0.05).astype(np.float32).astype(np.float64)
     y = normal(stream(6, 7), M)
     return dict(name="linear_d20_adaptive_n%d_m%d" % (g.size, M), basis=capi.BASIS_LINEAR,
                 grid=g, X=X, y=y, lam=1e-5, steps=steps)
+
+
+def regular_size(d, level):
+    """number of points of the regular no-boundary sparse grid"""
+    return sum(2 ** k * math.comb(d - 1 + k, d - 1) for k in range(level))
+
+
+def config_serving(n_models=64, rate=2000.0, duration=4.0, qmax=100000, nmin=1000, nmax=500000,
+                   seed=5):
+    """64 resident models (d uniform in 2..20; regular or adaptive Linear /
+    ModLinear grids with N log-uniform in [nmin, nmax]; alpha ~ N(0,1)) and a
+    Poisson request stream at `rate` req/s for `duration` seconds, each request
+    for a uniform model with q log-uniform in [1, qmax] points uniform on
+    [0,1]^d. Query points are slices of one uniform pool per dimension."""
+    from . import capi
+    u = uniform(stream(seed, 0), n_models * 5).reshape(n_models, 5)
+    models = []
+    for m in range(n_models):
+        d = 2 + min(18, int(u[m, 0] * 19))
+        basis = capi.BASIS_LINEAR if u[m, 1] < 0.5 else capi.BASIS_MODLINEAR
+        adaptive = u[m, 2] >= 0.5
+        target = math.exp(math.log(nmin) + u[m, 3] * (math.log(nmax) - math.log(nmin)))
+        levels = [L for L in range(1, 31) if regular_size(d, L) <= nmax]
+        if adaptive:
+            # regular start, then surplus refinement driven by a random indicator,
+            # with steps small enough not to overshoot the target by more than 2d points
+            L = max([L for L in levels if regular_size(d, L) <= target / 2] or [1])
+            g = capi.Grid(basis, d).regular(L)
+            step = 0
+            while g.size < target:
+                n0 = g.size
+                num = max(1, min(n0 // 5, int(math.ceil((target - n0) / (2 * d)))))
+                g.refine_surplus(1.0 + uniform(stream(seed * 1000 + m, 16 + step), n0), num)
+                step += 1
+                if g.size == n0:
+                    break
+        else:
+            inside = [L for L in levels if regular_size(d, L) >= nmin] or levels[-1:]
+            L = min(inside, key=lambda L: abs(math.log(regular_size(d, L)) - math.log(target)))
+            g = capi.Grid(basis, d).regular(L)
+        lv, ix = g.level_index()
+        alpha = normal(stream(seed * 1000 + m, 2), g.size)
+        models.append(dict(basis=basis, dim=d, level=lv, index=ix, alpha=alpha,
+                           kind="adaptive" if adaptive else "regular"))
+    n_req = int(rate * duration * 1.5) + 16
+    v = uniform(stream(seed, 3), 4 * n_req).reshape(n_req, 4)
+    t = np.cumsum(-np.log(1.0 - v[:, 0]) / rate)
+    keep = t < duration
+    t = t[keep]
+    model = np.minimum((v[keep, 1] * n_models).astype(np.int64), n_models - 1)
+    q = np.maximum(1, np.minimum(qmax, np.floor(np.exp(v[keep, 2] * math.log(qmax))))).astype(np.int64)
+    pool_rows = qmax + 4096
+    dims = sorted({mm["dim"] for mm in models})
+    pools = {d: uniform(stream(seed, 8 + d), pool_rows * d).reshape(pool_rows, d) for d in dims}
+    offs = (v[keep, 3] * (pool_rows - q)).astype(np.int64)
+    return dict(models=models, arrival=t, model=model, q=q, offset=offs, pools=pools,
+                rate=rate, duration=duration)

@@ -104,6 +104,29 @@ def test_bad_arguments():
     assert e.value.status_name == "INVALID_ARGUMENT"
 
 
+def test_refinement_stops_at_level_31():
+    """Refining the left-most point of a 1-D grid again and again: the
+    left-most point reaches level 31 after 30 steps; the next step would need
+    level 32 and is refused without touching the grid."""
+    g = capi.Grid.linear(1).regular(1)
+    for step in range(30):
+        lv, ix = g.level_index()
+        left = int(np.argmin(ix[:, 0] / 2.0 ** lv[:, 0]))
+        a = np.zeros(g.size)
+        a[left] = 1.0
+        g.refine_surplus(a, 1)
+    lv, ix = g.level_index()
+    assert lv.max() == 31 and g.size == 61
+    left = int(np.argmin(ix[:, 0] / 2.0 ** lv[:, 0]))
+    assert lv[left, 0] == 31 and ix[left, 0] == 1
+    a = np.zeros(g.size)
+    a[left] = 1.0
+    with pytest.raises(capi.SGppHIPError) as e:
+        g.refine_surplus(a, 1)
+    assert e.value.status_name == "GENERATION"
+    assert g.size == 61
+
+
 @ref_only
 @pytest.mark.parametrize("basis", [0, 1])
 @pytest.mark.parametrize("d,L", [(1, 5), (2, 4), (5, 5), (10, 4), (16, 3)])

@@ -86,3 +86,29 @@ def test_mixed_precision_refinement(gpu):
     op64 = capi.MultiEval(0, X).prepare(lv, ix)
     with pytest.raises(capi.SGppHIPError):
         op64.solve(y, lam, mode=capi.SOLVE_MIXED)
+
+
+def test_regression_workload_against_reference_solve(gpu):
+    """Linear d=8 level 6, 500,000 points, lambda 1e-4, epsilon 1e-10: the
+    FP64 GPU solve and the mixed solve against alpha of the reference's own
+    CPU solve (tests/golden/regression_alpha.npz, made by
+    tests/golden/make_regression_fixture.py). The FP64 GPU solve runs the
+    reference's CG recurrence, so its iteration count matches within 1; both
+    alphas agree with the reference's to 1e-8 relative."""
+    import os
+    capi = gpu
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "regression_alpha.npz"))
+    c = workloads.config_regression()
+    lv, ix = c["grid"].level_index()
+    op = capi.MultiEval(0, c["X"], precision=1, keep_fp64=True).prepare(lv, ix)
+    a64, i64 = op.solve(c["y"], c["lam"], mode=capi.SOLVE_FP64, max_iterations=10000,
+                        epsilon=1e-10)
+    amx, imx = op.solve(c["y"], c["lam"], mode=capi.SOLVE_MIXED, max_iterations=10000,
+                        epsilon=1e-10, inner_epsilon=1e-5, max_refinements=40)
+    print("reference: it %d; fp64: it %d rel %.2e; mixed: refinements %d matvec32 %d rel %.2e"
+          % (int(z["iterations"]), i64["iterations"], rel(a64, z["alpha"]), imx["refinements"],
+             imx["fp32_matvecs"], rel(amx, z["alpha"])), flush=True)
+    assert abs(i64["iterations"] - int(z["iterations"])) <= 1
+    assert rel(a64, z["alpha"]) <= 1e-8
+    assert imx["residual_rel"] <= 2e-10
+    assert rel(amx, z["alpha"]) <= 1e-8
