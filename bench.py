@@ -41,9 +41,18 @@ def parse():
     ap.add_argument("--serve", action="store_true",
                     help="run the served-loop workload (64 models, Poisson requests) instead")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rate", type=float, default=2000.0,
+                    help="Poisson arrival rate of the served workload (requests/s)")
     ap.add_argument("--solve", action="store_true",
                     help="run the regression-solve workload (FP64 CG vs mixed precision) instead")
     ap.add_argument("--inner-epsilon", type=float, default=1e-5)
+    ap.add_argument("--large", action="store_true",
+                    help="run the large streaming workload (Linear d=20, N>=1e6, M=1e7) instead")
+    ap.add_argument("--grid-target-large", type=int, default=1000000)
+    ap.add_argument("--data-points-large", type=int, default=10000000)
+    ap.add_argument("--copy-reference-mib", type=int, default=0,
+                    help="first copy this many MiB device to device three times (a kernel "
+                         "of known traffic for scaling hardware byte counters)")
     ap.add_argument("--grid-target", type=int, default=100000)
     ap.add_argument("--data-points", type=int, default=1000000)
     return ap.parse_args()
@@ -195,7 +204,7 @@ def serve_bench(args):
 
     from sgpp_amd import capi, workloads
     t = time.perf_counter()
-    c = workloads.config_serving(duration=args.serve_seconds)
+    c = workloads.config_serving(rate=args.serve_rate, duration=args.serve_seconds)
     gen_s = time.perf_counter() - t
     N = np.array([m["level"].shape[0] for m in c["models"]])
     D = np.array([m["dim"] for m in c["models"]])
@@ -305,6 +314,97 @@ def solve_bench(args):
     print(json.dumps(line))
 
 
+def large_bench(args):
+    """Large streaming case: Linear d=20 grid refined to >= 1,000,000 points,
+    10,000,000 FP32-representable mixture points, 5 CG iterations (lambda
+    1e-5) with FP32 products and FP64 vectors. Reports kernel TFLOP/s (6d per
+    pair) over every product launch of the solve, the dominant kernel's
+    roofline, and the FP32 error against FP64 GPU products on a 100,000-point
+    subsample whose FP64 results are checked against the oracle on a few rows
+    and grid points."""
+    import numpy as np
+
+    import oracle
+    from sgpp_amd import capi, workloads
+    t_start = time.perf_counter()
+
+    def note(msg):
+        print("[large] %s (%.0f s)" % (msg, time.perf_counter() - t_start), file=sys.stderr,
+              flush=True)
+
+    t = time.perf_counter()
+    c = workloads.config_large_streaming(
+        target=args.grid_target_large, M=args.data_points_large,
+        progress=lambda k, n: note("refinement step %d: %d grid points" % (k, n)))
+    gen_s = time.perf_counter() - t
+    note("workload ready")
+    lv, ix = c["grid"].level_index()
+    X, y, lam = c["X"], c["y"], c["lam"]
+    M, N, d = X.shape[0], lv.shape[0], lv.shape[1]
+    t = time.perf_counter()
+    op = capi.MultiEval(capi.BASIS_LINEAR, X, precision=capi.PREC_FP32)
+    op.prepare(lv, ix)
+    setup_s = time.perf_counter() - t
+    note("operator prepared; 5-iteration CG")
+    op.kernel_timing_enable(True)
+    alpha, info = op.solve(y, lam, mode=capi.SOLVE_FP32, max_iterations=5, epsilon=0.0)
+    nm, msm, nt, mst = op.kernel_timing()
+    op.kernel_timing_enable(False)
+    launches = nm + nt
+    flop_launch = 6.0 * d * M * N
+    kern_s = (msm + mst) * 1e-3
+    dominant = "multT_kernel" if mst / nt >= msm / nm else "mult_kernel"
+    dom_ms = max(mst / nt, msm / nm)
+    note("CG done; subsample check")
+    # subsample check: FP32 vs FP64 GPU products, FP64 vs the oracle
+    S = min(M, 100000)
+    sub32 = capi.MultiEval(capi.BASIS_LINEAR, X[:S], precision=capi.PREC_FP32).prepare(lv, ix)
+    sub64 = capi.MultiEval(capi.BASIS_LINEAR, X[:S], precision=capi.PREC_FP64).prepare(lv, ix)
+    a_test = workloads.normal(workloads.stream(6, 9), N)
+    y_test = y[:S]
+    r32, r64 = sub32.mult(a_test), sub64.mult(a_test)
+    t32, t64 = sub32.mult_transpose(y_test), sub64.mult_transpose(y_test)
+    rows = np.arange(0, S, S // 64)[:64]
+    cols = np.arange(0, N, N // 16)[:16]
+    o_r = oracle.mult(0, X[rows], lv, ix, a_test)
+    o_t = oracle.mult_transpose(0, X[:S], lv[cols], ix[cols], y_test)
+    line = {
+        "metric": "kernel TFLOP/s (6d per pair) over a 5-iteration CG, FP32 products",
+        "value": launches * flop_launch / kern_s / 1e12,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "fp32 products, fp64 accumulation and CG vectors",
+        "data": "synthetic, seeded (splitmix64): adaptive Linear d=20 grid, Gaussian-mixture points",
+        "config": {"workload": c["name"], "grid_points": N, "data_points": M, "dim": d,
+                   "lambda": lam, "cg_iterations": info["iterations"],
+                   "refinement_steps": c["steps"]},
+        "pair_evals_per_s": launches * M * N / kern_s,
+        "product_launches": {"mult": nm, "multT": nt},
+        "kernel_ms_avg": {"mult": msm / nm, "multT": mst / nt},
+        "solve_wall_s": info["seconds"],
+        "roofline": {"bound": "valu", "kernel": dominant,
+                     "achieved": flop_launch / (dom_ms * 1e-3) / 1e12,
+                     "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
+                     "frac": flop_launch / (dom_ms * 1e-3) / 1e12 / PEAK_TFLOPS["fp32"],
+                     "traffic": None, "flops_per_launch": flop_launch},
+        "subsample": {
+            "points": S,
+            "fp32_vs_fp64_mult_rel_l2": float(np.linalg.norm(r32 - r64) / np.linalg.norm(r64)),
+            "fp32_vs_fp64_multT_rel_l2": float(np.linalg.norm(t32 - t64) / np.linalg.norm(t64)),
+            "fp64_vs_oracle_mult_rel_l2": float(np.linalg.norm(r64[rows] - o_r) /
+                                                np.linalg.norm(o_r)),
+            "fp64_vs_oracle_multT_rel_l2": float(np.linalg.norm(t64[cols] - o_t) /
+                                                 np.linalg.norm(o_t)),
+            "oracle_sample": "%d rows (mult), %d grid points (multTranspose)" % (len(rows),
+                                                                                 len(cols)),
+        },
+        "generation_s": gen_s,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -316,6 +416,9 @@ def main():
         return
     if args.solve:
         solve_bench(args)
+        return
+    if args.large:
+        large_bench(args)
         return
     import numpy as np
     import torch
@@ -335,6 +438,13 @@ def main():
     op = capi.MultiEval(capi.BASIS_MODLINEAR, X, precision=capi.PREC_FP32, keep_fp64=True)
     op.prepare(lv, ix)
     setup_s = time.perf_counter() - t_setup
+    if args.copy_reference_mib > 0:
+        src = torch.ones(args.copy_reference_mib << 18, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     res = {}
     outs = {}

@@ -152,7 +152,12 @@ sgpp_hip_status sgpp_hip_meval_destroy(sgpp_hip_meval* h);
  * SystemMatrixLeastSquaresIdentity.cpp:32-56) solved by
  * ConjugateGradients::solve (solver/src/sgpp/solver/sle/ConjugateGradients.cpp:22-151).
  */
-enum { SGPP_HIP_SOLVE_FP64 = 0, SGPP_HIP_SOLVE_MIXED = 1 };
+/* FP64:  every product in FP64 (the reference's arithmetic);
+ * MIXED: FP64 iterative refinement around CG on FP32 products, stopping on
+ *        the FP64 residual ||b - A alpha|| <= epsilon ||b||;
+ * FP32:  the reference's CG with FP32 products (b included) and FP64
+ *        vectors, no refinement. */
+enum { SGPP_HIP_SOLVE_FP64 = 0, SGPP_HIP_SOLVE_MIXED = 1, SGPP_HIP_SOLVE_FP32 = 2 };
 
 typedef struct sgpp_hip_solve_params {
   int mode;                 /* SGPP_HIP_SOLVE_* */
@@ -174,7 +179,8 @@ typedef struct sgpp_hip_solve_info {
 } sgpp_hip_solve_info;
 
 /* y: M values, alpha: N values (in: start value when reuse, out: solution).
- * The operator must serve FP64 (FP64 mode) or FP32 and FP64 (mixed mode). */
+ * The operator must serve FP64 (FP64 mode), FP32 and FP64 (mixed mode) or
+ * FP32 (FP32 mode; residual_rel is then FP64 only if FP64 data is held). */
 sgpp_hip_status sgpp_hip_regression_solve(sgpp_hip_meval* h, const double* y,
                                           const sgpp_hip_solve_params* p, double* alpha,
                                           sgpp_hip_solve_info* info);
@@ -245,6 +251,98 @@ sgpp_hip_status sgpp_hip_server_cancel(sgpp_hip_server* s, uint64_t request_id);
 sgpp_hip_status sgpp_hip_server_inject_error(sgpp_hip_server* s);
 sgpp_hip_status sgpp_hip_server_drain(sgpp_hip_server* s);
 sgpp_hip_status sgpp_hip_server_stats(sgpp_hip_server* s, sgpp_hip_server_statistics* out);
+
+/* --------------------------------------------------------- learner
+ * The regression learner that drives the path: regular grid -> (refine ->
+ * prepare -> b = B^T y -> CG)* (LearnerBase::train, datadriven/src/sgpp/
+ * datadriven/application/LearnerBase.cpp:113-327, with
+ * LearnerLeastSquaresIdentity.cpp:32-39 for the system), predict
+ * (LearnerLeastSquaresIdentity.cpp:61-68) and getAccuracy
+ * (LearnerBase.cpp:369-411). Only CG is provided; error-based refinement
+ * is refused (SGPP_HIP_ERR_NOT_IMPLEMENTED).
+ */
+typedef struct sgpp_hip_learner sgpp_hip_learner;
+
+/* RegularGridConfiguration (base/src/sgpp/base/grid/Grid.hpp) */
+typedef struct sgpp_hip_grid_config {
+  int basis;  /* SGPP_HIP_BASIS_* */
+  size_t dim;
+  int level;
+} sgpp_hip_grid_config;
+
+/* SLESolverConfiguration (solver/src/sgpp/solver/TypesSolver.hpp:19-25) */
+enum { SGPP_HIP_SLE_CG = 0, SGPP_HIP_SLE_BICGSTAB = 1 };
+typedef struct sgpp_hip_solver_config {
+  int type;               /* SGPP_HIP_SLE_*; only CG is provided */
+  double eps;
+  size_t max_iterations;
+  double threshold;
+} sgpp_hip_solver_config;
+
+/* AdaptivityConfiguration (base/src/sgpp/base/grid/Grid.hpp:140-165) */
+typedef struct sgpp_hip_adaptivity_config {
+  size_t num_refinements;
+  double refinement_threshold;
+  size_t num_refinement_points;
+  int error_based_refinement;  /* must be 0 */
+} sgpp_hip_adaptivity_config;
+
+/* LearnerTiming (datadriven/src/sgpp/datadriven/tools/TypesDatadriven.hpp:37-54) */
+typedef struct sgpp_hip_learner_timing {
+  double time_complete;
+  double time_mult_complete;
+  double time_mult_compute;
+  double time_mult_trans_complete;
+  double time_mult_trans_compute;
+  double time_regularization;
+  double gflop;
+  double gbyte;
+} sgpp_hip_learner_timing;
+
+/* one pass of the refinement loop */
+typedef struct sgpp_hip_learner_step {
+  size_t grid_size;
+  size_t iterations;
+  size_t refinements;
+  double residual;        /* ||b - A alpha|| / ||b|| after the solve */
+  double train_accuracy;  /* MSE or accuracy when tested, else -1 */
+  double seconds;
+} sgpp_hip_learner_step;
+
+/* params selects the operator's precision (FP32_FP64ACC => mixed solve) and device */
+sgpp_hip_status sgpp_hip_learner_create(int is_regression, const sgpp_hip_params* params,
+                                        sgpp_hip_learner** out);
+sgpp_hip_status sgpp_hip_learner_destroy(sgpp_hip_learner* l);
+/* data: M x d row-major, classes: M values */
+sgpp_hip_status sgpp_hip_learner_train(sgpp_hip_learner* l, const double* data, size_t M,
+                                       size_t d, const double* classes,
+                                       const sgpp_hip_grid_config* grid,
+                                       const sgpp_hip_solver_config* solver_refine,
+                                       const sgpp_hip_solver_config* solver_final,
+                                       const sgpp_hip_adaptivity_config* adaptivity,
+                                       int test_acc_during_adapt, double lambda,
+                                       sgpp_hip_learner_timing* timing);
+sgpp_hip_status sgpp_hip_learner_predict(sgpp_hip_learner* l, const double* data, size_t M,
+                                         size_t d, double* out);
+sgpp_hip_status sgpp_hip_learner_accuracy(sgpp_hip_learner* l, const double* data, size_t M,
+                                          size_t d, const double* classes, double threshold,
+                                          double* accuracy);
+sgpp_hip_status sgpp_hip_learner_grid_size(sgpp_hip_learner* l, size_t* n);
+/* level/index: N x d, alpha: N */
+sgpp_hip_status sgpp_hip_learner_level_index(sgpp_hip_learner* l, uint32_t* level,
+                                             uint32_t* index);
+sgpp_hip_status sgpp_hip_learner_alpha(sgpp_hip_learner* l, double* alpha);
+sgpp_hip_status sgpp_hip_learner_num_steps(sgpp_hip_learner* l, size_t* n);
+sgpp_hip_status sgpp_hip_learner_step_info(sgpp_hip_learner* l, size_t k,
+                                           sgpp_hip_learner_step* out);
+
+/* GFlop / GByte of one CG run on a grid under the reference's convention
+ * (LearnerVectorizedPerformanceCalculator::getGFlopAndGByte,
+ * datadriven/src/sgpp/datadriven/tools/LearnerVectorizedPerformanceCalculator.cpp:17-113) */
+sgpp_hip_status sgpp_hip_learner_performance(sgpp_hip_grid* grid, size_t num_instances,
+                                             int solver_type, size_t num_iterations,
+                                             size_t size_datatype, int reuse_alpha,
+                                             double* gflop, double* gbyte);
 
 #ifdef __cplusplus
 }

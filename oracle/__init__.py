@@ -177,6 +177,12 @@ def ref():
             R.ref_op_seconds.restype = ctypes.c_double
             R.ref_cg_solve.argtypes = [_vp, _sz, _sz, _dp, ctypes.c_double, _sz, ctypes.c_double,
                                        _dp, ctypes.POINTER(_sz), ctypes.POINTER(ctypes.c_double)]
+            R.ref_gflop_gbyte.argtypes = [_vp, _sz, ctypes.c_int, _sz, _sz, ctypes.c_int, _dp, _dp]
+            R.ref_learner_train.argtypes = [_vp, ctypes.c_int, _dp, _sz, _dp, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_double, _sz,
+                                            ctypes.c_double, _sz, _sz, ctypes.c_double, _sz,
+                                            ctypes.c_int, _dp, _sz, ctypes.POINTER(_sz),
+                                            ctypes.POINTER(_sz), ctypes.POINTER(_sz), _dp]
             _ref, _ref_variant = R, v
             return R
     return None
@@ -238,6 +244,45 @@ class RefGrid:
         out = np.empty_like(v)
         _rcheck(ref().ref_grid_hierarchise(self.h, _d(v), _d(out)))
         return out
+
+    def performance(self, num_instances, iterations, solver=0, size_datatype=8,
+                    reuse_alpha=False):
+        """the reference's LearnerVectorizedPerformanceCalculator::getGFlopAndGByte"""
+        gf, gb = ctypes.c_double(0), ctypes.c_double(0)
+        _rcheck(ref().ref_gflop_gbyte(self.h, num_instances, solver, iterations, size_datatype,
+                                      int(reuse_alpha), ctypes.byref(gf), ctypes.byref(gb)))
+        return gf.value, gb.value
+
+
+def learner_train(basis, X, y, level, lam, refine=(1e-10, 1000), final=None, num_refinements=1,
+                  threshold=0.0, num_points=5, test_acc=False, kind=1):
+    """LearnerBase::train's steps on reference parts (ref_learner_train).
+    Returns (RefGrid, alpha, [(grid size, CG iterations, train MSE or -1)])."""
+    X = _f64(X)
+    y = _f64(y)
+    M, d = X.shape
+    final = refine if final is None else final
+    g = RefGrid(basis, d)
+    cap = 4 * (regular_size_bound(d, level) + (num_refinements + 1) * num_points * (2 * d + 2) ** 2)
+    alpha = np.empty(cap)
+    k = num_refinements + 1
+    sizes = np.zeros(k, dtype=np.uint64)
+    iters = np.zeros(k, dtype=np.uint64)
+    mse = np.zeros(k)
+    n = _sz(0)
+    P = ctypes.POINTER(_sz)
+    _rcheck(ref().ref_learner_train(g.h, kind, _d(X), M, _d(y), level, lam, refine[0],
+                                    refine[1], final[0], final[1], num_refinements, threshold,
+                                    num_points, int(test_acc), _d(alpha), cap, ctypes.byref(n),
+                                    sizes.ctypes.data_as(P), iters.ctypes.data_as(P), _d(mse)))
+    steps = [(int(sizes[j]), int(iters[j]), float(mse[j])) for j in range(n.value)]
+    return g, alpha[:g.size].copy(), steps
+
+
+def regular_size_bound(d, level):
+    """points of the regular no-boundary grid of this level"""
+    import math
+    return sum(2 ** k * math.comb(d - 1 + k, d - 1) for k in range(level))
 
 
 class RefOp:

@@ -24,6 +24,7 @@
 #include <sgpp/datadriven/operation/hash/OperationMultiEvalModMaskStreaming/OperationMultiEvalModMaskStreaming.hpp>
 #include <sgpp/datadriven/operation/hash/OperationMultiEvalStreaming/OperationMultiEvalStreaming.hpp>
 #include <sgpp/datadriven/operation/hash/OperationMultipleEvalSubspace/combined/OperationMultipleEvalSubspaceCombined.hpp>
+#include <sgpp/datadriven/tools/LearnerVectorizedPerformanceCalculator.hpp>
 #include <sgpp/solver/sle/ConjugateGradients.hpp>
 
 #include <chrono>
@@ -206,6 +207,90 @@ int ref_cg_solve(void* o, size_t M, size_t N, const double* y, double lambda, si
     std::memcpy(alpha, a.getPointer(), N * sizeof(double));
     if (iterations) *iterations = cg.getNumberIterations();
     if (residuum) *residuum = cg.getResiduum();
+  });
+}
+
+// LearnerVectorizedPerformanceCalculator::getGFlopAndGByte, unchanged
+int ref_gflop_gbyte(void* g, size_t M, int solver, size_t iterations, size_t size_dt, int reuse,
+                    double* gflop, double* gbyte) {
+  return guard([&] {
+    auto p = sgpp::datadriven::LearnerVectorizedPerformanceCalculator::getGFlopAndGByte(
+        *static_cast<RefGrid*>(g)->grid, M,
+        solver == 0 ? sgpp::solver::SLESolverType::CG : sgpp::solver::SLESolverType::BiCGSTAB,
+        iterations, size_dt, reuse != 0, false);
+    *gflop = p.GFlop_;
+    *gbyte = p.GByte_;
+  });
+}
+
+// The steps of LearnerBase::train (LearnerBase.cpp:113-327) for a regression
+// learner with surplus refinement, on reference parts: the regular grid of
+// InitializeGrid (:81-98), the operator kind chosen as in ref_op_new, the
+// system of SystemMatrixLeastSquaresIdentity (LeastSquares above; b = B^T y),
+// ConjugateGradients with reused coefficients (LearnerBase.cpp:30) and
+// SurplusRefinementFunctor + refine. g: an empty grid from ref_grid_new.
+// Per pass: grid size, CG iterations, train MSE (when test_acc). Returns the
+// passes in *n_steps (arrays sized num_refinements + 1) and alpha in
+// alpha_out (capacity alpha_cap, at least the final grid size).
+int ref_learner_train(void* g, int kind, const double* X, size_t M, const double* y, int level,
+                      double lambda, double eps_refine, size_t it_refine, double eps_final,
+                      size_t it_final, size_t num_refinements, double threshold,
+                      size_t num_points, int test_acc, double* alpha_out, size_t alpha_cap,
+                      size_t* n_steps, size_t* step_size, size_t* step_iterations,
+                      double* step_mse) {
+  return guard([&] {
+    Grid& grid = *static_cast<RefGrid*>(g)->grid;
+    const size_t d = grid.getDimension();
+    grid.getGenerator().regular(level);
+    DataVector alpha(grid.getSize());
+    alpha.setAll(0.0);
+    DataMatrix data(X, M, d);
+    DataVector classes(M);
+    std::memcpy(classes.getPointer(), y, M * sizeof(double));
+    std::unique_ptr<sgpp::base::OperationMultipleEval> B;
+    if (kind == 0)
+      B.reset(sgpp::op_factory::createOperationMultipleEval(grid, data));
+    else if (grid.getType() == sgpp::base::GridType::Linear)
+      B.reset(new sgpp::datadriven::OperationMultiEvalStreaming(grid, data));
+    else
+      B.reset(new sgpp::datadriven::OperationMultiEvalModMaskStreaming(grid, data));
+    LeastSquares A(*B, M, lambda);
+    sgpp::solver::ConjugateGradients cg(it_refine, eps_refine);
+    double oldAcc = 0.0;
+    size_t k = 0;
+    for (size_t i = 0; i < num_refinements + 1; ++i) {
+      if (i > 0) {
+        sgpp::base::SurplusRefinementFunctor f(alpha, num_points, threshold);
+        grid.getGenerator().refine(f);
+        B->prepare();
+        alpha.resizeZero(grid.getSize());
+      }
+      DataVector b(alpha.getSize());
+      DataVector myClasses(classes);
+      B->multTranspose(myClasses, b);
+      if (i == num_refinements) {
+        cg.setMaxIterations(it_final);
+        cg.setEpsilon(eps_final);
+      }
+      cg.solve(A, alpha, b, true, false, 0.0);
+      step_size[k] = grid.getSize();
+      step_iterations[k] = cg.getNumberIterations();
+      step_mse[k] = -1.0;
+      ++k;
+      if (test_acc) {
+        DataVector computed(M);
+        B->mult(alpha, computed);
+        computed.sub(classes);
+        computed.sqr();
+        const double acc = computed.sum() / static_cast<double>(M);
+        step_mse[k - 1] = acc;
+        if (i > 0 && oldAcc <= acc) break;
+        oldAcc = acc;
+      }
+    }
+    *n_steps = k;
+    if (alpha.getSize() > alpha_cap) throw std::runtime_error("alpha_out too small");
+    std::memcpy(alpha_out, alpha.getPointer(), alpha.getSize() * sizeof(double));
   });
 }
 

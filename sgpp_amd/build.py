@@ -17,7 +17,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["meval_kernels.hip", "solver_kernels.hip", "server.hip"]
-CPP_SOURCES = ["grid.cpp", "operation_hip.cpp", "capi.cpp", "server_capi.cpp"]
+CPP_SOURCES = ["grid.cpp", "operation_hip.cpp", "learner.cpp", "capi.cpp", "server_capi.cpp"]
 
 INCLUDES = ["-I" + CSRC, "-I" + os.path.join(CSRC, "include"),
             "-I" + os.path.join(os.path.dirname(HERE), "include")]

@@ -3,7 +3,9 @@
 Python mirror of the C++ operator interface for tests and benchmarks:
 ``Grid`` (createLinearGrid / createModLinearGrid, regular, refine,
 hierarchise), ``MultiEval`` (OperationMultipleEval: mult, multTranspose,
-prepare, getDuration) and ``regression_solve``. There is no fallback: if the
+prepare, getDuration, the regression solve), ``Server`` (resident models,
+continuous batching) and ``Learner`` (LearnerLeastSquaresIdentity: train,
+predict, getAccuracy). There is no fallback: if the
 library is missing, importing this module raises.
 """
 import ctypes
@@ -20,7 +22,8 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "FACTORY", 3: "OPERATION", 4:
                 10: "TIMEOUT", 11: "SOLVER", 99: "INTERNAL"}
 BASIS_LINEAR, BASIS_MODLINEAR = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
-SOLVE_FP64, SOLVE_MIXED = 0, 1
+SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
+SLE_CG, SLE_BICGSTAB = 0, 1
 
 
 class SGppHIPError(RuntimeError):
@@ -73,6 +76,36 @@ class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_size_t), ("refinements", ctypes.c_size_t),
                 ("fp64_matvecs", ctypes.c_size_t), ("fp32_matvecs", ctypes.c_size_t),
                 ("residual_rel", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+class GridConfig(ctypes.Structure):
+    _fields_ = [("basis", ctypes.c_int), ("dim", ctypes.c_size_t), ("level", ctypes.c_int)]
+
+
+class SolverConfig(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_int), ("eps", ctypes.c_double),
+                ("max_iterations", ctypes.c_size_t), ("threshold", ctypes.c_double)]
+
+
+class AdaptivityConfig(ctypes.Structure):
+    _fields_ = [("num_refinements", ctypes.c_size_t), ("refinement_threshold", ctypes.c_double),
+                ("num_refinement_points", ctypes.c_size_t),
+                ("error_based_refinement", ctypes.c_int)]
+
+
+class LearnerTiming(ctypes.Structure):
+    _fields_ = [("time_complete", ctypes.c_double), ("time_mult_complete", ctypes.c_double),
+                ("time_mult_compute", ctypes.c_double),
+                ("time_mult_trans_complete", ctypes.c_double),
+                ("time_mult_trans_compute", ctypes.c_double),
+                ("time_regularization", ctypes.c_double), ("gflop", ctypes.c_double),
+                ("gbyte", ctypes.c_double)]
+
+
+class LearnerStep(ctypes.Structure):
+    _fields_ = [("grid_size", ctypes.c_size_t), ("iterations", ctypes.c_size_t),
+                ("refinements", ctypes.c_size_t), ("residual", ctypes.c_double),
+                ("train_accuracy", ctypes.c_double), ("seconds", ctypes.c_double)]
 
 
 def _decl(name, restype, argtypes):
@@ -139,6 +172,21 @@ SIGNATURES = {
     "sgpp_hip_server_inject_error": (_st, [_p]),
     "sgpp_hip_server_drain": (_st, [_p]),
     "sgpp_hip_server_stats": (_st, [_p, ctypes.c_void_p]),
+    "sgpp_hip_learner_create": (_st, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_p)]),
+    "sgpp_hip_learner_destroy": (_st, [_p]),
+    "sgpp_hip_learner_train": (_st, [_p, _dp, _sz, _sz, _dp, ctypes.POINTER(GridConfig),
+                                     ctypes.POINTER(SolverConfig), ctypes.POINTER(SolverConfig),
+                                     ctypes.POINTER(AdaptivityConfig), ctypes.c_int,
+                                     ctypes.c_double, ctypes.POINTER(LearnerTiming)]),
+    "sgpp_hip_learner_predict": (_st, [_p, _dp, _sz, _sz, _dp]),
+    "sgpp_hip_learner_accuracy": (_st, [_p, _dp, _sz, _sz, _dp, ctypes.c_double, _dp]),
+    "sgpp_hip_learner_grid_size": (_st, [_p, ctypes.POINTER(_sz)]),
+    "sgpp_hip_learner_level_index": (_st, [_p, _up, _up]),
+    "sgpp_hip_learner_alpha": (_st, [_p, _dp]),
+    "sgpp_hip_learner_num_steps": (_st, [_p, ctypes.POINTER(_sz)]),
+    "sgpp_hip_learner_step_info": (_st, [_p, _sz, ctypes.POINTER(LearnerStep)]),
+    "sgpp_hip_learner_performance": (_st, [_p, _sz, ctypes.c_int, _sz, _sz, ctypes.c_int, _dp,
+                                           _dp]),
 }
 
 F = {}
@@ -246,6 +294,17 @@ class Grid:
         check(F["sgpp_hip_grid_unserialize"](text.encode(), ctypes.byref(h)))
         basis = BASIS_LINEAR if text.lstrip().startswith("linear") else BASIS_MODLINEAR
         return cls(basis, 0, _handle=h)
+
+
+    def performance(self, num_instances, iterations, solver=SLE_CG, size_datatype=8,
+                    reuse_alpha=False):
+        """(GFlop, GByte) of one CG run under the reference learners' convention
+        (LearnerVectorizedPerformanceCalculator::getGFlopAndGByte)"""
+        gf, gb = ctypes.c_double(0), ctypes.c_double(0)
+        check(F["sgpp_hip_learner_performance"](self.h, num_instances, solver, iterations,
+                                                size_datatype, int(reuse_alpha),
+                                                ctypes.byref(gf), ctypes.byref(gb)))
+        return gf.value, gb.value
 
 
 class MultiEval:
@@ -459,3 +518,93 @@ class Server:
         s = ServerStatistics()
         check(F["sgpp_hip_server_stats"](self.h, ctypes.cast(ctypes.byref(s), ctypes.c_void_p)))
         return {k: getattr(s, k) for k, _ in ServerStatistics._fields_}
+
+
+# ------------------------------------------------------------- learner
+
+class Learner:
+    """LearnerLeastSquaresIdentity on the GPU operator (sgpp_hip_learner_*)."""
+
+    def __init__(self, is_regression=True, precision=PREC_FP64, device=0):
+        prm = Params(precision, 0, device, 0, 0, 0)
+        h = _p()
+        check(F["sgpp_hip_learner_create"](int(is_regression), ctypes.byref(prm),
+                                           ctypes.byref(h)))
+        self.h = h
+        self.d = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and F is not None:
+            F["sgpp_hip_learner_destroy"](h)
+            self.h = None
+
+    def close(self):
+        self.__del__()
+
+    def train(self, X, y, basis, level, lam, refine=(1e-10, 1000), final=None,
+              num_refinements=1, refinement_threshold=0.0, num_refinement_points=5,
+              test_acc_during_adapt=False, error_based_refinement=False, solver=SLE_CG):
+        """refine / final: (eps, max_iterations) of the CG runs during and
+        after refinement; returns the LearnerTiming fields as a dict"""
+        X = _f64(X)
+        y = _f64(y)
+        M, d = X.shape
+        if y.shape != (M,):
+            # LearnerBase::train: "length of classes vector does not match to dataset!"
+            raise ValueError("y must have one value per data point")
+        final = refine if final is None else final
+        gc = GridConfig(basis, d, level)
+        sr = SolverConfig(solver, refine[0], refine[1], 0.0)
+        sf = SolverConfig(solver, final[0], final[1], 0.0)
+        ac = AdaptivityConfig(num_refinements, refinement_threshold, num_refinement_points,
+                              int(error_based_refinement))
+        t = LearnerTiming()
+        check(F["sgpp_hip_learner_train"](self.h, _dptr(X), M, d, _dptr(y), ctypes.byref(gc),
+                                          ctypes.byref(sr), ctypes.byref(sf), ctypes.byref(ac),
+                                          int(test_acc_during_adapt), lam, ctypes.byref(t)))
+        self.d = d
+        return {k: getattr(t, k) for k, _ in LearnerTiming._fields_}
+
+    def predict(self, X):
+        X = _f64(X)
+        out = np.empty(X.shape[0])
+        check(F["sgpp_hip_learner_predict"](self.h, _dptr(X), X.shape[0], X.shape[1], _dptr(out)))
+        return out
+
+    def accuracy(self, X, y, threshold=0.0):
+        X = _f64(X)
+        y = _f64(y)
+        a = ctypes.c_double(0)
+        check(F["sgpp_hip_learner_accuracy"](self.h, _dptr(X), X.shape[0], X.shape[1], _dptr(y),
+                                             threshold, ctypes.byref(a)))
+        return a.value
+
+    @property
+    def grid_size(self):
+        n = _sz(0)
+        check(F["sgpp_hip_learner_grid_size"](self.h, ctypes.byref(n)))
+        return n.value
+
+    def level_index(self):
+        n = self.grid_size
+        lv = np.empty((n, self.d), dtype=np.uint32)
+        ix = np.empty((n, self.d), dtype=np.uint32)
+        check(F["sgpp_hip_learner_level_index"](self.h, lv.ctypes.data_as(_up),
+                                                ix.ctypes.data_as(_up)))
+        return lv, ix
+
+    def alpha(self):
+        a = np.empty(self.grid_size)
+        check(F["sgpp_hip_learner_alpha"](self.h, _dptr(a)))
+        return a
+
+    def steps(self):
+        n = _sz(0)
+        check(F["sgpp_hip_learner_num_steps"](self.h, ctypes.byref(n)))
+        out = []
+        for k in range(n.value):
+            st = LearnerStep()
+            check(F["sgpp_hip_learner_step_info"](self.h, k, ctypes.byref(st)))
+            out.append({f: getattr(st, f) for f, _ in LearnerStep._fields_})
+        return out

@@ -11,6 +11,7 @@
 #include "../../include/sgpp_hip.h"
 #include "meval_device.hpp"
 #include "sgpp/base/grid/Grid.hpp"
+#include "sgpp/datadriven/application/LearnerLeastSquaresIdentity.hpp"
 #include "sgpp/datadriven/operation/OperationMultiEvalHIP.hpp"
 #include "solver_device.hpp"
 
@@ -21,6 +22,10 @@ using sgpp::base::Grid;
 struct sgpp_hip_grid {
   std::unique_ptr<Grid> grid;
   std::unique_ptr<sgpp::base::IncrementalHierarchisation> hier;
+};
+
+struct sgpp_hip_learner {
+  std::unique_ptr<sgpp::datadriven::LearnerLeastSquaresIdentity> learner;
 };
 
 struct sgpp_hip_meval {
@@ -60,6 +65,8 @@ sgpp_hip_status guarded(F&& f) {
     return fail(SGPP_HIP_ERR_OPERATION, e.what());
   } catch (const sgpp::base::solver_exception& e) {
     return fail(SGPP_HIP_ERR_SOLVER, e.what());
+  } catch (const sgpp::base::application_exception& e) {
+    return fail(SGPP_HIP_ERR_INVALID_ARGUMENT, e.what());
   } catch (const sgpp_hip::input_error& e) {
     return fail(SGPP_HIP_ERR_INVALID_ARGUMENT, e.what());
   } catch (const sgpp_hip::device_error& e) {
@@ -77,6 +84,22 @@ sgpp_hip::Precision prec_of(int p) {
   if (p == SGPP_HIP_PREC_FP64) return sgpp_hip::Precision::FP64;
   if (p == SGPP_HIP_PREC_FP32_FP64ACC) return sgpp_hip::Precision::FP32;
   throw sgpp::base::data_exception("unknown precision");
+}
+
+sgpp::datadriven::HIPOperationConfiguration hip_config_of(const sgpp_hip_params* params) {
+  sgpp::datadriven::HIPOperationConfiguration hc;
+  if (params) {
+    prec_of(params->precision);
+    hc.precision = params->precision == SGPP_HIP_PREC_FP64
+                       ? sgpp::datadriven::HIPPrecision::FP64
+                       : sgpp::datadriven::HIPPrecision::FP32_FP64ACC;
+    hc.keepFP64 = params->keep_fp64 != 0;
+    hc.device = params->device;
+    hc.multGridChunks = params->mult_grid_chunks;
+    hc.multTDataSegs = params->multT_data_segs;
+    hc.targetBlocks = params->target_blocks;
+  }
+  return hc;
 }
 
 sgpp::base::GridType basis_of(int b) {
@@ -280,18 +303,7 @@ sgpp_hip_status sgpp_hip_meval_create_for_grid(sgpp_hip_grid* grid, const double
     auto h = std::unique_ptr<sgpp_hip_meval>(new sgpp_hip_meval());
     const size_t d = grid->grid->getDimension();
     h->data = DataMatrix(data, M, d);
-    sgpp::datadriven::HIPOperationConfiguration hc;
-    if (params) {
-      hc.precision = params->precision == SGPP_HIP_PREC_FP64
-                         ? sgpp::datadriven::HIPPrecision::FP64
-                         : sgpp::datadriven::HIPPrecision::FP32_FP64ACC;
-      prec_of(params->precision);
-      hc.keepFP64 = params->keep_fp64 != 0;
-      hc.device = params->device;
-      hc.multGridChunks = params->mult_grid_chunks;
-      hc.multTDataSegs = params->multT_data_segs;
-      hc.targetBlocks = params->target_blocks;
-    }
+    const sgpp::datadriven::HIPOperationConfiguration hc = hip_config_of(params);
     sgpp::datadriven::OperationMultipleEvalConfiguration cfg(
         sgpp::datadriven::OperationMultipleEvalType::STREAMING,
         sgpp::datadriven::OperationMultipleEvalSubType::HIP, hc, "HIP");
@@ -446,14 +458,18 @@ sgpp_hip_status sgpp_hip_regression_solve(sgpp_hip_meval* h, const double* y,
                                           sgpp_hip_solve_info* info) {
   return guarded([&] {
     REQUIRE(h && y && p && alpha, "null argument");
-    REQUIRE(p->mode == SGPP_HIP_SOLVE_FP64 || p->mode == SGPP_HIP_SOLVE_MIXED, "unknown mode");
-    REQUIRE(h->dev->serves(sgpp_hip::Precision::FP64),
-            "the solver needs an operator that holds FP64 data (precision FP64 or keep_fp64)");
-    if (p->mode == SGPP_HIP_SOLVE_MIXED)
+    REQUIRE(p->mode == SGPP_HIP_SOLVE_FP64 || p->mode == SGPP_HIP_SOLVE_MIXED ||
+                p->mode == SGPP_HIP_SOLVE_FP32,
+            "unknown mode");
+    if (p->mode != SGPP_HIP_SOLVE_FP32)
+      REQUIRE(h->dev->serves(sgpp_hip::Precision::FP64),
+              "the solver needs an operator that holds FP64 data (precision FP64 or keep_fp64)");
+    if (p->mode != SGPP_HIP_SOLVE_FP64)
       REQUIRE(h->dev->serves(sgpp_hip::Precision::FP32),
-              "mixed mode needs an operator that holds FP32 data");
+              "this mode needs an operator that holds FP32 data");
     sgpp_hip::SolveOptions o;
     o.mixed = p->mode == SGPP_HIP_SOLVE_MIXED;
+    o.fp32_products = p->mode == SGPP_HIP_SOLVE_FP32;
     o.lambda = p->lambda;
     o.max_iterations = p->max_iterations;
     o.epsilon = p->epsilon;
@@ -480,6 +496,186 @@ sgpp_hip_status sgpp_hip_system_matrix_mult(sgpp_hip_meval* h, double lambda, co
     REQUIRE(h && v && q, "null argument");
     sgpp_hip::DeviceRegression solver(*h->dev);
     solver.system_mult_host(v, q, lambda, prec_of(precision));
+    return SGPP_HIP_OK;
+  });
+}
+
+// ---------------------------------------------------------------- learner
+
+namespace {
+
+sgpp::solver::SLESolverConfiguration solver_config_of(const sgpp_hip_solver_config* c) {
+  sgpp::solver::SLESolverConfiguration s;
+  if (c->type == SGPP_HIP_SLE_CG)
+    s.type_ = sgpp::solver::SLESolverType::CG;
+  else if (c->type == SGPP_HIP_SLE_BICGSTAB)
+    s.type_ = sgpp::solver::SLESolverType::BiCGSTAB;
+  else
+    throw sgpp::base::data_exception("unknown solver type");
+  s.eps_ = c->eps;
+  s.maxIterations_ = c->max_iterations;
+  s.threshold_ = c->threshold;
+  return s;
+}
+
+}  // namespace
+
+sgpp_hip_status sgpp_hip_learner_create(int is_regression, const sgpp_hip_params* params,
+                                        sgpp_hip_learner** out) {
+  return guarded([&] {
+    REQUIRE(out, "out is null");
+    auto l = std::unique_ptr<sgpp_hip_learner>(new sgpp_hip_learner());
+    l->learner.reset(new sgpp::datadriven::LearnerLeastSquaresIdentity(is_regression != 0));
+    l->learner->setImplementation(sgpp::datadriven::OperationMultipleEvalConfiguration(
+        sgpp::datadriven::OperationMultipleEvalType::STREAMING,
+        sgpp::datadriven::OperationMultipleEvalSubType::HIP, hip_config_of(params), "learner"));
+    *out = l.release();
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_destroy(sgpp_hip_learner* l) {
+  return guarded([&] {
+    delete l;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_train(sgpp_hip_learner* l, const double* data, size_t M,
+                                       size_t d, const double* classes,
+                                       const sgpp_hip_grid_config* grid,
+                                       const sgpp_hip_solver_config* solver_refine,
+                                       const sgpp_hip_solver_config* solver_final,
+                                       const sgpp_hip_adaptivity_config* adaptivity,
+                                       int test_acc_during_adapt, double lambda,
+                                       sgpp_hip_learner_timing* timing) {
+  return guarded([&] {
+    REQUIRE(l && data && classes && grid && solver_refine && solver_final && adaptivity,
+            "null argument");
+    REQUIRE(M > 0 && d >= 1 && d <= 64, "bad data shape");
+    DataMatrix X(data, M, d);
+    DataVector y(classes, M);
+    sgpp::base::RegularGridConfiguration gc;
+    gc.type_ = basis_of(grid->basis);
+    gc.dim_ = grid->dim;
+    gc.level_ = grid->level;
+    sgpp::base::AdaptivityConfiguration ac;
+    ac.numRefinements_ = adaptivity->num_refinements;
+    ac.refinementThreshold_ = adaptivity->refinement_threshold;
+    ac.numRefinementPoints_ = adaptivity->num_refinement_points;
+    ac.errorBasedRefinement_ = adaptivity->error_based_refinement != 0;
+    const sgpp::datadriven::LearnerTiming t = l->learner->train(
+        X, y, gc, solver_config_of(solver_refine), solver_config_of(solver_final), ac,
+        test_acc_during_adapt != 0, lambda);
+    if (timing) {
+      timing->time_complete = t.timeComplete_;
+      timing->time_mult_complete = t.timeMultComplete_;
+      timing->time_mult_compute = t.timeMultCompute_;
+      timing->time_mult_trans_complete = t.timeMultTransComplete_;
+      timing->time_mult_trans_compute = t.timeMultTransCompute_;
+      timing->time_regularization = t.timeRegularization_;
+      timing->gflop = t.GFlop_;
+      timing->gbyte = t.GByte_;
+    }
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_predict(sgpp_hip_learner* l, const double* data, size_t M,
+                                         size_t d, double* out) {
+  return guarded([&] {
+    REQUIRE(l && data && out, "null argument");
+    REQUIRE(l->learner->isTrained(), "learner is not trained");
+    REQUIRE(d == l->learner->getGrid().getDimension(), "data dimension differs from the grid's");
+    DataMatrix X(data, M, d);
+    DataVector r;
+    l->learner->predict(X, r);
+    std::memcpy(out, r.data(), M * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_accuracy(sgpp_hip_learner* l, const double* data, size_t M,
+                                          size_t d, const double* classes, double threshold,
+                                          double* accuracy) {
+  return guarded([&] {
+    REQUIRE(l && data && classes && accuracy, "null argument");
+    REQUIRE(l->learner->isTrained(), "learner is not trained");
+    REQUIRE(d == l->learner->getGrid().getDimension(), "data dimension differs from the grid's");
+    DataMatrix X(data, M, d);
+    DataVector y(classes, M);
+    *accuracy = l->learner->getAccuracy(X, y, threshold);
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_grid_size(sgpp_hip_learner* l, size_t* n) {
+  return guarded([&] {
+    REQUIRE(l && n, "null argument");
+    *n = l->learner->isTrained() ? l->learner->getGrid().getSize() : 0;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_level_index(sgpp_hip_learner* l, uint32_t* level,
+                                             uint32_t* index) {
+  return guarded([&] {
+    REQUIRE(l && level && index, "null argument");
+    REQUIRE(l->learner->isTrained(), "learner is not trained");
+    std::vector<uint32_t> lv, ix;
+    l->learner->getGrid().getStorage().getLevelIndexArrays(lv, ix);
+    std::memcpy(level, lv.data(), lv.size() * sizeof(uint32_t));
+    std::memcpy(index, ix.data(), ix.size() * sizeof(uint32_t));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_alpha(sgpp_hip_learner* l, double* alpha) {
+  return guarded([&] {
+    REQUIRE(l && alpha, "null argument");
+    REQUIRE(l->learner->isTrained(), "learner is not trained");
+    const DataVector& a = l->learner->getAlpha();
+    std::memcpy(alpha, a.data(), a.size() * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_num_steps(sgpp_hip_learner* l, size_t* n) {
+  return guarded([&] {
+    REQUIRE(l && n, "null argument");
+    *n = l->learner->getSteps().size();
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_step_info(sgpp_hip_learner* l, size_t k,
+                                           sgpp_hip_learner_step* out) {
+  return guarded([&] {
+    REQUIRE(l && out, "null argument");
+    const auto& steps = l->learner->getSteps();
+    REQUIRE(k < steps.size(), "step index out of range");
+    out->grid_size = steps[k].gridSize;
+    out->iterations = steps[k].iterations;
+    out->refinements = steps[k].refinements;
+    out->residual = steps[k].residual;
+    out->train_accuracy = steps[k].trainAccuracy;
+    out->seconds = steps[k].seconds;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_learner_performance(sgpp_hip_grid* grid, size_t num_instances,
+                                             int solver_type, size_t num_iterations,
+                                             size_t size_datatype, int reuse_alpha,
+                                             double* gflop, double* gbyte) {
+  return guarded([&] {
+    REQUIRE(grid && gflop && gbyte, "null argument");
+    sgpp_hip_solver_config c{solver_type, 0.0, 0, 0.0};
+    const auto p = sgpp::datadriven::LearnerVectorizedPerformanceCalculator::getGFlopAndGByte(
+        *grid->grid, num_instances, solver_config_of(&c).type_, num_iterations, size_datatype,
+        reuse_alpha != 0);
+    *gflop = p.GFlop_;
+    *gbyte = p.GByte_;
     return SGPP_HIP_OK;
   });
 }

@@ -327,8 +327,12 @@ void GridGenerator::refine(RefinementFunctor& functor, std::vector<size_t>* adde
   std::vector<ref_pair> collection;
   HashGridPoint point(st.getDimension());
   for (auto it = st.begin(); it != st.end(); ++it) {
+    if (it->first->childrenComplete()) continue;
     point = *(it->first);
-    if (!has_missing_child(st, point)) continue;
+    if (!has_missing_child(st, point)) {
+      it->first->setChildrenComplete();
+      continue;
+    }
     collection.push_back(ref_pair(it->second, functor(st, it->second)));
     std::push_heap(collection.begin(), collection.end(), compare_pairs);
     if (collection.size() > num) {
@@ -364,8 +368,12 @@ size_t GridGenerator::getNumberOfRefinablePoints() {
   size_t c = 0;
   HashGridPoint point(storage_.getDimension());
   for (auto it = storage_.begin(); it != storage_.end(); ++it) {
+    if (it->first->childrenComplete()) continue;
     point = *(it->first);
-    if (has_missing_child(storage_, point)) ++c;
+    if (has_missing_child(storage_, point))
+      ++c;
+    else
+      it->first->setChildrenComplete();
   }
   return c;
 }

@@ -34,6 +34,10 @@ struct generation_exception : public sgpp_exception {
 struct solver_exception : public sgpp_exception {
   explicit solver_exception(const std::string& m = "solver_exception") : sgpp_exception(m) {}
 };
+struct application_exception : public sgpp_exception {
+  explicit application_exception(const std::string& m = "application_exception")
+      : sgpp_exception(m) {}
+};
 
 }  // namespace base
 }  // namespace sgpp

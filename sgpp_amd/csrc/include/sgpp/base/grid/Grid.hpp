@@ -39,38 +39,37 @@ class HashGridPoint {
   HashGridPoint() = default;
   explicit HashGridPoint(size_t dim) : level_(dim, 0), index_(dim, 0) {}
   HashGridPoint(const HashGridPoint& o) : level_(o.level_), index_(o.index_), leaf_(o.leaf_) {
-    rehash();
+    copy_hash(o);
   }
   HashGridPoint& operator=(const HashGridPoint& o) {
     if (this != &o) {
       level_ = o.level_;
       index_ = o.index_;
       leaf_ = o.leaf_;
-      rehash();
+      copy_hash(o);
     }
     return *this;
   }
 
   size_t getDimension() const { return level_.size(); }
   void set(size_t d, level_type l, index_type i) {
+    const size_t before = term(d);
     level_[d] = l;
     index_[d] = i;
-    rehash();
+    update_hash(d, before);
   }
   void set(size_t d, level_type l, index_type i, bool isLeaf) {
-    level_[d] = l;
-    index_[d] = i;
     leaf_ = isLeaf;
-    rehash();
+    set(d, l, i);
   }
   // no rehash (HashGridPoint.hpp:125-143); callers copy the point afterwards
   void push(size_t d, level_type l, index_type i) {
     level_[d] = l;
     index_[d] = i;
+    hash_valid_ = false;
   }
   void push(size_t d, level_type l, index_type i, bool isLeaf) {
-    level_[d] = l;
-    index_[d] = i;
+    push(d, l, i);
     leaf_ = isLeaf;
   }
   void get(size_t d, level_type& l, index_type& i) const {
@@ -81,6 +80,11 @@ class HashGridPoint {
   index_type getIndex(size_t d) const { return index_[d]; }
   void setLeaf(bool b) { leaf_ = b; }
   bool isLeaf() const { return leaf_; }
+  // Set by refinement on a stored point once all 2d children are in its
+  // storage; points are never removed one by one, so it stays true. Not
+  // copied: a copy may live in a storage without those children.
+  bool childrenComplete() const { return children_complete_; }
+  void setChildrenComplete() { children_complete_ = true; }
   size_t getHash() const { return hash_; }
   bool equals(const HashGridPoint& o) const {
     return level_ == o.level_ && index_ == o.index_;
@@ -98,20 +102,53 @@ class HashGridPoint {
   double getStandardCoordinate(size_t d) const {
     return static_cast<double>(index_[d]) / static_cast<double>(uint64_t(1) << level_[d]);
   }
+  // HashGridPoint.cpp:151-178: h = 0xdeadbeef; h = (2^l_d + i_d) + h * 65599
+  // over d = 0..dim-1 (the level-index sum in 32 bits, h in 64 bits)
   void rehash() {
     size_t h = 0xdeadbeef;
-    for (size_t d = 0; d < level_.size(); ++d) {
-      const index_type hinv = static_cast<index_type>(1) << level_[d];
-      h = hinv + index_[d] + h * 65599;
-    }
+    for (size_t d = 0; d < level_.size(); ++d) h = term(d) + h * kHashMul;
     hash_ = h;
+    hash_valid_ = true;
   }
 
  private:
+  static constexpr size_t kHashMul = 65599;
+  static constexpr size_t kMaxIncrementalDim = 64;
+  // the hash is a polynomial in kHashMul, so changing the term of dimension
+  // d changes it by (new - old) * kHashMul^(dim - 1 - d) modulo 2^64: set()
+  // updates it in O(1) and yields the value rehash() would compute
+  static size_t hash_power(size_t k) {
+    static const std::vector<size_t> table = [] {
+      std::vector<size_t> t(kMaxIncrementalDim, 1);
+      for (size_t j = 1; j < t.size(); ++j) t[j] = t[j - 1] * kHashMul;
+      return t;
+    }();
+    return table[k];
+  }
+  size_t term(size_t d) const {
+    return static_cast<index_type>((static_cast<index_type>(1) << level_[d]) + index_[d]);
+  }
+  void update_hash(size_t d, size_t before) {
+    if (hash_valid_ && level_.size() <= kMaxIncrementalDim)
+      hash_ += (term(d) - before) * hash_power(level_.size() - 1 - d);
+    else
+      rehash();
+  }
+  void copy_hash(const HashGridPoint& o) {
+    if (o.hash_valid_) {
+      hash_ = o.hash_;
+      hash_valid_ = true;
+    } else {
+      rehash();
+    }
+  }
+
   std::vector<level_type> level_;
   std::vector<index_type> index_;
   size_t hash_ = 0;
+  bool hash_valid_ = false;
   bool leaf_ = false;
+  bool children_complete_ = false;
 };
 
 // Not noexcept on purpose: matches the reference functors, which makes

@@ -34,6 +34,12 @@ struct input_error : public std::runtime_error {
 
 void check_hip(hipError_t e, const char* what);
 
+// hipGetLastError() also returns an error that an earlier HIP call left
+// behind in this thread, including calls of other code in the process that
+// ignores its own errors. Every launch here is preceded by this call, so the
+// check after the launch reports that launch only.
+inline void clear_launch_error() { (void)hipGetLastError(); }
+
 // Launch tuning knobs; 0 means "choose automatically".
 struct LaunchConfig {
   int mult_grid_chunks = 0;   // G: grid-point chunks for mult (partials over G)

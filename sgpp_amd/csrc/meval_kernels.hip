@@ -320,6 +320,7 @@ void launch_mult_D(bool skip, const T* X, int64_t ldx, const T* LI, const T* A,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
+  clear_launch_error();
   if (skip)
     hipLaunchKernelGGL((mult_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, tm,
                        part, chunk, jend);
@@ -335,6 +336,7 @@ void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
   constexpr int K = multT_k<T>(D);
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
   dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
+  clear_launch_error();
   if (skip)
     hipLaunchKernelGGL((multT_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI, tm,
                        part, ldp, seg, mend);
@@ -558,6 +560,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   hipStream_t s = stream_;
   const int32_t* perm = skip_ ? static_cast<const int32_t*>(perm_) : nullptr;
   ensure(&A_, &A_cap_, Npad_ * sizeof(T));
+  clear_launch_error();
   hipLaunchKernelGGL(prep_alpha_kernel<T>, dim3(ceil_div(Npad_, 256)), dim3(256), 0, s, d_alpha,
                      scale_, perm, static_cast<T*>(A_), static_cast<int64_t>(N_),
                      static_cast<int64_t>(Npad_));
@@ -585,12 +588,14 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
       static_cast<const uint32_t*>(tmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
+    clear_launch_error();
     hipLaunchKernelGGL(mult_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
                        static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_), P, chunk, jend,
                        static_cast<int>(D_));
   }
   check_hip(hipGetLastError(), "mult kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
+  clear_launch_error();
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M_, 256)), dim3(256), 0, s, P,
                      static_cast<int64_t>(Mpad_), G, static_cast<const double*>(nullptr),
                      static_cast<const int32_t*>(nullptr), d_result, static_cast<int64_t>(M_));
@@ -603,6 +608,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
                                 hipEvent_t* tev) {
   hipStream_t s = stream_;
   ensure(&Y_, &Y_cap_, Mpad_ * sizeof(T));
+  clear_launch_error();
   hipLaunchKernelGGL(prep_y_kernel<T>, dim3(ceil_div(Mpad_, 256)), dim3(256), 0, s, d_y,
                      static_cast<T*>(Y_), static_cast<int64_t>(M_), static_cast<int64_t>(Mpad_));
   check_hip(hipGetLastError(), "prep_y");
@@ -635,12 +641,14 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
       static_cast<int64_t>(N_), s);
   } else {
     dim3 grid(static_cast<unsigned>(ceil_div(N_, kBlock)), static_cast<unsigned>(S));
+    clear_launch_error();
     hipLaunchKernelGGL(multT_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
                        static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, P,
                        static_cast<int64_t>(Npad_), seg, mend, static_cast<int>(D_));
   }
   check_hip(hipGetLastError(), "multTranspose kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
+  clear_launch_error();
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N_, 256)), dim3(256), 0, s, P,
                      static_cast<int64_t>(Npad_), S, scale_,
                      skip_ ? static_cast<const int32_t*>(perm_) : nullptr, d_result,

@@ -287,6 +287,7 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
       hipGraph_t g;
       check_hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal),
                 "hipStreamBeginCapture");
+      clear_launch_error();
       hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s->stream, s->dCounter);
       if (cfg_.precision == Precision::FP64)
         hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s->stream,
@@ -612,6 +613,7 @@ void Server::launch_batch(Slot& s) {
   if (s.graph) {
     check_hip(hipGraphLaunch(s.graph, s.stream), "hipGraphLaunch");
   } else {
+    clear_launch_error();
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s.stream, s.dCounter);
     if (cfg_.precision == Precision::FP64)
       hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s.stream,

@@ -9,6 +9,8 @@ namespace sgpp_hip {
 
 struct SolveOptions {
   bool mixed = false;
+  // with mixed == false: CG with FP32 products (b included), FP64 vectors
+  bool fp32_products = false;
   double lambda = 0.0;
   size_t max_iterations = 1000;
   double epsilon = 1e-10;

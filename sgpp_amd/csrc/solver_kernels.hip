@@ -117,6 +117,7 @@ DeviceRegression::~DeviceRegression() {
 void DeviceRegression::apply(const double* v, double* q, double lambda, Precision p) {
   op_.mult_device(v, tempM_, p);
   op_.mult_transpose_device(tempM_, q, p);
+  clear_launch_error();
   hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(),
                      static_cast<double>(M_) * lambda, v, q, static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "axpy");
@@ -130,6 +131,7 @@ double DeviceRegression::dot(const double* x, const double* y) {
   const int64_t n = static_cast<int64_t>(N_);
   const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
   double* part = static_cast<double*>(dotpart_);
+  clear_launch_error();
   hipLaunchKernelGGL(dot_stage1, dim3(kDotBlocks), dim3(kRB), 0, op_.stream(), x, y, n, chunk,
                      part);
   hipLaunchKernelGGL(dot_stage2, dim3(1), dim3(kRB), 0, op_.stream(), part, kDotBlocks,
@@ -143,21 +145,25 @@ double DeviceRegression::dot(const double* x, const double* y) {
 }
 
 void DeviceRegression::axpy(double a, const double* x, double* y) {
+  clear_launch_error();
   hipLaunchKernelGGL(axpy_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), a, x, y,
                      static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "axpy");
 }
 void DeviceRegression::xpby(const double* x, double b, double* y) {
+  clear_launch_error();
   hipLaunchKernelGGL(xpby_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), x, b, y,
                      static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "xpby");
 }
 void DeviceRegression::sub(const double* x, const double* y, double* z) {
+  clear_launch_error();
   hipLaunchKernelGGL(sub_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), x, y, z,
                      static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "sub");
 }
 void DeviceRegression::fill(double* y, double v) {
+  clear_launch_error();
   hipLaunchKernelGGL(fill_kernel, dim3(blocks_for(N_)), dim3(kRB), 0, op_.stream(), y, v,
                      static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "fill");
@@ -221,11 +227,15 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
   const auto t0 = std::chrono::steady_clock::now();
   fp64_matvecs_ = fp32_matvecs_ = 0;
   SolveResult res;
-  // b = B^T y in FP64 (SystemMatrixLeastSquaresIdentity::generateb)
+  const bool plain32 = !o.mixed && o.fp32_products;
+  const Precision bp = plain32 ? Precision::FP32 : Precision::FP64;
+  // reporting residual: FP64 when the operator holds FP64 data
+  const Precision rp = op_.serves(Precision::FP64) ? Precision::FP64 : Precision::FP32;
+  // b = B^T y (SystemMatrixLeastSquaresIdentity::generateb)
   check_hip(hipMemcpyAsync(tempM_, y_host, M_ * sizeof(double), hipMemcpyHostToDevice,
                            op_.stream()),
             "copy y");
-  op_.mult_transpose_device(tempM_, b_, Precision::FP64);
+  op_.mult_transpose_device(tempM_, b_, bp);
   if (o.reuse)
     check_hip(hipMemcpyAsync(alpha_, alpha_host, N_ * sizeof(double), hipMemcpyHostToDevice,
                              op_.stream()),
@@ -233,8 +243,8 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
   const double bnorm = std::sqrt(dot(b_, b_));
   if (!o.mixed) {
     double fd = 0.0, sd = 0.0;
-    res.iterations = cg(alpha_, b_, o.lambda, Precision::FP64, o.max_iterations, o.epsilon,
-                        o.reuse, &fd, &sd, false);
+    res.iterations = cg(alpha_, b_, o.lambda, bp, o.max_iterations, o.epsilon, o.reuse, &fd,
+                        &sd, false);
   } else {
     // r = b - A64 alpha (alpha = 0 or reused)
     // the outer residual lives in c_, which cg() does not touch
@@ -271,12 +281,16 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
     check_hip(hipFree(corr), "hipFree");
     check_hip(hipFree(rhs), "hipFree");
   }
-  // final FP64 residual ||b - A alpha|| / ||b||
-  apply(alpha_, q_, o.lambda, Precision::FP64);
+  // final residual ||b - A alpha|| / ||b||
+  apply(alpha_, q_, o.lambda, rp);
   sub(b_, q_, c_);
   const double rn = std::sqrt(dot(c_, c_));
   res.residual_rel = bnorm > 0 ? rn / bnorm : rn;
-  --fp64_matvecs_;  // the reporting residual is not part of the solve
+  // the reporting residual is not part of the solve
+  if (rp == Precision::FP64)
+    --fp64_matvecs_;
+  else
+    --fp32_matvecs_;
   check_hip(hipMemcpyAsync(alpha_host, alpha_, N_ * sizeof(double), hipMemcpyDeviceToHost,
                            op_.stream()),
             "copy alpha out");

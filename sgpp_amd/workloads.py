@@ -93,9 +93,10 @@ def damped_cosine_product(c):
     return np.prod(4.0 * c * (1.0 - c) * (1.0 + 0.5 * np.cos(np.pi * c)), axis=1)
 
 
-def adaptive_grid(basis, dim, start_level, f, per_step, target, max_steps=10000):
+def adaptive_grid(basis, dim, start_level, f, per_step, target, max_steps=10000, progress=None):
     """Regular grid, then repeat {hierarchise f at the grid points; refine the
     per_step largest |surplus|} until the grid has >= target points.
+    progress(steps, size) is called every 10 steps when given.
     Returns (grid, alpha_of_f, steps)."""
     from . import capi
     g = capi.Grid(basis, dim).regular(start_level)
@@ -113,6 +114,8 @@ def adaptive_grid(basis, dim, start_level, f, per_step, target, max_steps=10000)
         vals = np.concatenate([vals, f(new)])
         alpha = g.hierarchise(vals, incremental=True)
         steps += 1
+        if progress is not None and steps % 10 == 0:
+            progress(steps, g.size)
     return g, alpha, steps
 
 
@@ -158,7 +161,7 @@ def config_regression(M=500000, level=6, dim=8):
                 basis=capi.BASIS_LINEAR, grid=g, X=X, y=y, lam=1e-4)
 
 
-def config_large_streaming(target=1000000, M=10000000):
+def config_large_streaming(target=1000000, M=10000000, progress=None):
     """Linear d=20: regular level 4, +200 refinements per step of the
     hierarchised damped cosine product until >= target points; M points from
     an 8-component mixture (sigma 0.05, means in [0.2,0.8]^20, clipped, seed 6),
@@ -166,7 +169,7 @@ def config_large_streaming(target=1000000, M=10000000):
     y ~ N(0,1) (seed 6, tag 7); lambda 1e-5."""
     from . import capi
     g, falpha, steps = adaptive_grid(capi.BASIS_LINEAR, 20, 4, damped_cosine_product, 200,
-                                     target)
+                                     target, progress=progress)
     X = gaussian_mixture(6, M, 20, 8, 0.05).astype(np.float32).astype(np.float64)
     y = normal(stream(6, 7), M)
     return dict(name="linear_d20_adaptive_n%d_m%d" % (g.size, M), basis=capi.BASIS_LINEAR,

@@ -112,3 +112,30 @@ def test_regression_workload_against_reference_solve(gpu):
     assert rel(a64, z["alpha"]) <= 1e-8
     assert imx["residual_rel"] <= 2e-10
     assert rel(amx, z["alpha"]) <= 1e-8
+
+
+def test_fp32_products_cg(gpu):
+    """The reference's CG on FP32 products (FP64 vectors): same iteration
+    semantics, accuracy limited by the FP32 products. Checked against the
+    FP64 solve: 1e-4 relative at a 1e-6 residual."""
+    capi = gpu
+    g, X, y = small_regression(capi, d=4, level=4, M=20000, seed=44)
+    lv, ix = g.level_index()
+    lam = 1e-4
+    op32 = capi.MultiEval(0, X, precision=1).prepare(lv, ix)
+    a32, i32 = op32.solve(y, lam, mode=capi.SOLVE_FP32, max_iterations=5000, epsilon=1e-6)
+    op64 = capi.MultiEval(0, X).prepare(lv, ix)
+    a64, i64 = op64.solve(y, lam, mode=capi.SOLVE_FP64, max_iterations=5000, epsilon=1e-13)
+    print("fp32 CG: it %d resid %.2e rel %.2e" % (i32["iterations"], i32["residual_rel"],
+                                                  rel(a32, a64)), flush=True)
+    # start residual + one per iteration + the residual recomputed every 50 iterations
+    it = i32["iterations"]
+    assert i32["fp64_matvecs"] == 0 and i32["fp32_matvecs"] == 1 + it + (it - 1) // 50
+    assert i32["residual_rel"] <= 1e-5
+    assert rel(a32, a64) <= 1e-4
+    # a fixed number of iterations (the large streaming case runs 5)
+    a5, i5 = op32.solve(y, lam, mode=capi.SOLVE_FP32, max_iterations=5, epsilon=0.0)
+    assert i5["iterations"] == 5 and i5["fp32_matvecs"] == 6
+    # an FP32-only operator cannot run the FP64 solve
+    with pytest.raises(capi.SGppHIPError):
+        op32.solve(y, lam, mode=capi.SOLVE_FP64)
