@@ -51,6 +51,9 @@ constexpr int kBlock = 256;  // 4 waves of 64
 constexpr int kTG = 64;      // grid points per LDS tile (mult) and per mask tile
 constexpr int kTM = 128;     // data points per LDS tile (multTranspose)
 constexpr int kMaxTemplD = 20;
+// default launch size: workgroups per CU and launch (measured optimum region
+// 40-300 on the d=10 grids; below ~16 the last wave of blocks leaves CUs idle)
+constexpr int kBlocksPerCU = 64;
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -59,12 +62,6 @@ __device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
 __device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
 __device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
 
-// first factor: max(0, 1 - |u|)
-template <typename T>
-__device__ __forceinline__ T hat_first(T L, T x, T I) {
-  const T u = fmaT(L, x, -I);
-  return clamp01(T(1) - absT(u));
-}
 // running product: p * max(0, 1 - |u|) for p in [0,1]
 template <typename T>
 __device__ __forceinline__ T hat_step(T p, T L, T x, T I) {
@@ -72,28 +69,49 @@ __device__ __forceinline__ T hat_step(T p, T L, T x, T I) {
   return clamp01(fmaT(-p, absT(u), p));
 }
 
+// Points per lane (K) and points per sub-tile (J). One wave-uniform branch
+// per (sub-tile, dimension) decides whether a dimension is evaluated, so the
+// branch cost is shared by J*K factor updates per lane. K keeps the per-lane
+// operands (coordinates for mult, (L, I) for multTranspose) within ~40 FP32
+// or ~80 FP64 registers. Values from launch-shape and variant sweeps on the
+// d=10 ModLinear headline grid.
 template <typename T>
 constexpr int mult_k(int D) {
-  return sizeof(T) == 8 ? (D <= 5 ? 8 : (D <= 10 ? 4 : (D <= 20 ? 2 : 1)))
-                        : (D <= 5 ? 16 : (D <= 10 ? 8 : 4));
+  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 5 ? 8 : (D <= 10 ? 4 : 2));
 }
 template <typename T>
-constexpr int multT_k(int D) {
-  return sizeof(T) == 8 ? (D <= 5 ? 4 : (D <= 10 ? 2 : 1)) : (D <= 5 ? 8 : (D <= 10 ? 4 : 2));
+constexpr int mult_j() {
+  return sizeof(T) == 8 ? 4 : 8;
+}
+// multTranspose: with the dimension skip one tile per wave (K = 1) keeps the
+// wave's mask as small as the tile's; without it two points per lane and
+// sub-tiles of 4 data points run fastest.
+template <typename T>
+constexpr int multT_k(int D, bool skip) {
+  return skip ? 1 : ((sizeof(T) == 8 && D > 10) ? 1 : 2);
+}
+constexpr int multT_j(bool skip) { return skip ? 8 : 4; }
+
+// (L, I) of a 64-point tile t and dimension d: LI[((t*D + d)*64 + jj)*2 + {0,1}]
+// (tile-major, then dimension, then point), so the J points of a sub-tile are
+// contiguous for every dimension and a wave's 64 points are contiguous too.
+__device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) {
+  return (((j >> 6) * D + d) * 64 + (j & 63)) * 2;
 }
 
 // ---------------------------------------------------------------- mult
-// X: [D][ldx] SoA, LI: [Npad][D][2], A: [Npad] (alpha*scale, zero padded),
-// tmask: [Npad/64] non-constant dimensions of each 64-point tile (kSkip only),
+// X: [D][ldx] SoA, LI: tile-major pairs (li_at), A: [Npad] (alpha*scale, zero
+// padded), tmask: [Npad/64] non-constant dimensions of each tile (kSkip only),
 // part: [G][ldx]. Block covers kBlock*K data points and grid chunk
 // [blockIdx.y*chunk, min(+chunk, jend_all)).
-template <int D, int K, typename T, bool kSkip>
+template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
                                                       const T* __restrict__ A,
                                                       const uint32_t* __restrict__ tmask,
                                                       double* __restrict__ part, int chunk,
                                                       int jend_all) {
+  static_assert(kTG % J == 0, "sub-tile must divide the tile");
   __shared__ T sLI[kTG * D * 2];
   __shared__ T sA[kTG];
   const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + threadIdx.x;
@@ -119,28 +137,30 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
-    for (int j = 0; j < kTG; ++j) {
-      // all (L, I) of the point first: one batch of LDS reads, one wait
-      T L[D], I[D];
+    for (int j = 0; j < kTG; j += J) {
+      T p[J][K];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        L[d] = sLI[(j * D + d) * 2];
-        I[d] = sLI[(j * D + d) * 2 + 1];
-      }
-      T p[K];
+      for (int q = 0; q < J; ++q)
 #pragma unroll
-      for (int k = 0; k < K; ++k) p[k] = T(1);
+        for (int k = 0; k < K; ++k) p[q][k] = T(1);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         if (kSkip && !((mask >> d) & 1u)) continue;
+        // (L, I) of the sub-tile's J points in dimension d: one batch of LDS reads
+        T li[2 * J];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          p[k] = (!kSkip && d == 0) ? hat_first(L[d], x[k][d], I[d])
-                                    : hat_step(p[k], L[d], x[k][d], I[d]);
+        for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
+#pragma unroll
+        for (int q = 0; q < J; ++q)
+#pragma unroll
+          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[2 * q], x[k][d], li[2 * q + 1]);
       }
-      const T a = sA[j];
 #pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[k], acc[k]);
+      for (int q = 0; q < J; ++q) {
+        const T a = sA[j + q];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[q][k], acc[k]);
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
@@ -165,9 +185,11 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
   for (int j0 = jb; j0 < je; j0 += kTG) {
     T acc = T(0);
     for (int j = j0; j < j0 + kTG; ++j) {
-      const T* li = LI + static_cast<int64_t>(j) * D * 2;
-      T p = hat_first(li[0], X[m], li[1]);
-      for (int d = 1; d < D; ++d) p = hat_step(p, li[2 * d], X[d * ldx + m], li[2 * d + 1]);
+      T p = T(1);
+      for (int d = 0; d < D; ++d) {
+        const int64_t e = li_at(j, d, D);
+        p = hat_step(p, LI[e], X[d * ldx + m], LI[e + 1]);
+      }
       acc = fmaT(A[j], p, acc);
     }
     accd += static_cast<double>(acc);
@@ -176,20 +198,20 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 }
 
 // ------------------------------------------------------- multTranspose
-// X: [D][ldx] SoA, Y: [ldx] (zero padded), LI: [Npad][D][2],
+// X: [D][ldx] SoA, Y: [ldx] (zero padded), LI: tile-major pairs (li_at),
 // part: [S][ldp]. Block covers grid points [blockIdx.x*kBlock*K, +kBlock*K)
-// and data segment [blockIdx.y*seg, min(+seg, mend)). With kSkip each wave
-// evaluates only the union of the non-constant dimensions of its K tiles.
-template <int D, int K, typename T, bool kSkip>
+// and data segment [blockIdx.y*seg, min(+seg, mend)). The data tile sits in
+// LDS as [D+1][kTM] (coordinates, then y). With kSkip each wave evaluates
+// only the union of the non-constant dimensions of its K tiles.
+template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
                                                        const uint32_t* __restrict__ tmask,
                                                        double* __restrict__ part, int64_t ldp,
                                                        int64_t seg, int64_t mend) {
-  constexpr int kVec = 16 / static_cast<int>(sizeof(T));
-  constexpr int RS = ((D + 1 + kVec - 1) / kVec) * kVec;  // LDS row stride
-  __shared__ T sX[kTM * RS];
+  static_assert(kTM % J == 0, "sub-tile must divide the tile");
+  __shared__ T sX[(D + 1) * kTM];
   const int64_t jblock = static_cast<int64_t>(blockIdx.x) * (kBlock * K);
   const int64_t j0 = jblock + threadIdx.x;
   uint32_t mask = 0xffffffffu;
@@ -204,8 +226,9 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      L[k][d] = LI[((j0 + k * kBlock) * D + d) * 2];
-      I[k][d] = LI[((j0 + k * kBlock) * D + d) * 2 + 1];
+      const int64_t e = li_at(j0 + k * kBlock, d, D);
+      L[k][d] = LI[e];
+      I[k][d] = LI[e + 1];
     }
   double accd[K];
 #pragma unroll
@@ -218,32 +241,35 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
     for (int e = threadIdx.x; e < kTM * (D + 1); e += kBlock) {
       const int d = e / kTM;
       const int m = e - d * kTM;
-      sX[m * RS + d] = (d < D) ? X[d * ldx + m0 + m] : Y[m0 + m];
+      sX[e] = (d < D) ? X[d * ldx + m0 + m] : Y[m0 + m];
     }
     __syncthreads();
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
-    for (int m = 0; m < kTM; ++m) {
-      const T* row = sX + m * RS;
-      // the whole row first: one batch of LDS reads, one wait
-      T xr[D + 1];
+    for (int m = 0; m < kTM; m += J) {
+      T p[J][K];
 #pragma unroll
-      for (int d = 0; d <= D; ++d) xr[d] = row[d];
-      T p[K];
+      for (int q = 0; q < J; ++q)
 #pragma unroll
-      for (int k = 0; k < K; ++k) p[k] = T(1);
+        for (int k = 0; k < K; ++k) p[q][k] = T(1);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         if (kSkip && !((mask >> d) & 1u)) continue;
+        T xv[J];
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-          p[k] = (!kSkip && d == 0) ? hat_first(L[k][d], xr[d], I[k][d])
-                                    : hat_step(p[k], L[k][d], xr[d], I[k][d]);
+        for (int q = 0; q < J; ++q) xv[q] = sX[d * kTM + m + q];
+#pragma unroll
+        for (int q = 0; q < J; ++q)
+#pragma unroll
+          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], L[k][d], xv[q], I[k][d]);
       }
-      const T y = xr[D];
 #pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = fmaT(y, p[k], acc[k]);
+      for (int q = 0; q < J; ++q) {
+        const T y = sX[D * kTM + m + q];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = fmaT(y, p[q][k], acc[k]);
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
@@ -261,15 +287,17 @@ __global__ __launch_bounds__(kBlock) void multT_kernel_generic(const T* __restri
                                                                int64_t ldp, int64_t seg,
                                                                int64_t mend, int D) {
   const int64_t j = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const T* li = LI + j * D * 2;
   const int64_t mb = static_cast<int64_t>(blockIdx.y) * seg;
   const int64_t me = min(mb + seg, mend);
   double accd = 0.0;
   for (int64_t m0 = mb; m0 < me; m0 += kTM) {
     T acc = T(0);
     for (int64_t m = m0; m < m0 + kTM; ++m) {
-      T p = hat_first(li[0], X[m], li[1]);
-      for (int d = 1; d < D; ++d) p = hat_step(p, li[2 * d], X[d * ldx + m], li[2 * d + 1]);
+      T p = T(1);
+      for (int d = 0; d < D; ++d) {
+        const int64_t e = li_at(j, d, D);
+        p = hat_step(p, LI[e], X[d * ldx + m], LI[e + 1]);
+      }
       acc = fmaT(Y[m], p, acc);
     }
     accd += static_cast<double>(acc);
@@ -319,30 +347,41 @@ void launch_mult_D(bool skip, const T* X, int64_t ldx, const T* LI, const T* A,
                    const uint32_t* tm, double* part, int chunk, int jend, int G, int64_t Mpad,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
+  constexpr int J = mult_j<T>();
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
-  if (skip)
-    hipLaunchKernelGGL((mult_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, tm,
-                       part, chunk, jend);
-  else
-    hipLaunchKernelGGL((mult_kernel<D, K, T, false>), grid, dim3(kBlock), 0, s, X, ldx, LI, A, tm,
-                       part, chunk, jend);
+  if constexpr (D >= 2) {
+    if (skip) {
+      hipLaunchKernelGGL((mult_kernel<D, K, J, T, true>), grid, dim3(kBlock), 0, s, X, ldx, LI, A,
+                         tm, part, chunk, jend);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((mult_kernel<D, K, J, T, false>), grid, dim3(kBlock), 0, s, X, ldx, LI, A,
+                     tm, part, chunk, jend);
 }
 
 template <typename T, int D>
 void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
                     const uint32_t* tm, double* part, int64_t ldp, int64_t seg, int64_t mend,
                     int S, int64_t N, hipStream_t s) {
-  constexpr int K = multT_k<T>(D);
+  clear_launch_error();
+  // one dimension leaves nothing to skip (and its skip variant compiles badly)
+  if constexpr (D >= 2) {
+    if (skip) {
+      constexpr int K = multT_k<T>(D, true);
+      const int64_t cover = static_cast<int64_t>(kBlock) * K;
+      dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
+      hipLaunchKernelGGL((multT_kernel<D, K, multT_j(true), T, true>), grid, dim3(kBlock), 0, s,
+                         X, ldx, Y, LI, tm, part, ldp, seg, mend);
+      return;
+    }
+  }
+  constexpr int K = multT_k<T>(D, false);
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
   dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-  clear_launch_error();
-  if (skip)
-    hipLaunchKernelGGL((multT_kernel<D, K, T, true>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI, tm,
-                       part, ldp, seg, mend);
-  else
-    hipLaunchKernelGGL((multT_kernel<D, K, T, false>), grid, dim3(kBlock), 0, s, X, ldx, Y, LI,
-                       tm, part, ldp, seg, mend);
+  hipLaunchKernelGGL((multT_kernel<D, K, multT_j(false), T, false>), grid, dim3(kBlock), 0, s, X,
+                     ldx, Y, LI, tm, part, ldp, seg, mend);
 }
 
 template <typename T>
@@ -373,8 +412,8 @@ int mult_k_rt(int D) {
   return D > kMaxTemplD ? 1 : mult_k<T>(D);
 }
 template <typename T>
-int multT_k_rt(int D) {
-  return D > kMaxTemplD ? 1 : multT_k<T>(D);
+int multT_k_rt(int D, bool skip) {
+  return D > kMaxTemplD ? 1 : multT_k<T>(D, skip && D >= 2);
 }
 
 int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
@@ -521,6 +560,17 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
   // padded points: (L, I) = (0, 0) -> factor 1, weight 0 through A / ignored rows
   li.resize(Npad * D_ * 2, 0.0);
   sc.resize(Npad, 1.0);
+  // point-major [Npad][D][2] -> tile-major [Npad/64][D][64][2]
+  {
+    std::vector<double> t(li.size());
+    for (size_t j = 0; j < Npad; ++j)
+      for (size_t d = 0; d < D_; ++d) {
+        const size_t dst = (((j / kTG) * D_ + d) * kTG + (j % kTG)) * 2;
+        t[dst] = li[(j * D_ + d) * 2];
+        t[dst + 1] = li[(j * D_ + d) * 2 + 1];
+      }
+    li.swap(t);
+  }
   std::vector<uint32_t> tm(Npad / kTG, 0u);
   for (size_t s = 0; s < N; ++s) tm[s / kTG] |= pmask[order[s]];
   check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -570,7 +620,9 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   const int bx = static_cast<int>(Mpad_ / (kBlock * K));
   int G = cfg_.mult_grid_chunks;
   if (G <= 0) {
-    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : 8 * num_cus_;
+    // many short blocks: the dispatcher then balances the uneven per-block
+    // work (the skip makes tiles cost different amounts) across all CUs
+    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : kBlocksPerCU * num_cus_;
     G = std::max(1, ceil_div(target, bx));
     G = std::min(G, std::max(1, ntiles / 4));  // at least 4 tiles per chunk
   }
@@ -612,7 +664,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
   hipLaunchKernelGGL(prep_y_kernel<T>, dim3(ceil_div(Mpad_, 256)), dim3(256), 0, s, d_y,
                      static_cast<T*>(Y_), static_cast<int64_t>(M_), static_cast<int64_t>(Mpad_));
   check_hip(hipGetLastError(), "prep_y");
-  const int K = multT_k_rt<T>(static_cast<int>(D_));
+  const int K = multT_k_rt<T>(static_cast<int>(D_), skip_);
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
   const int bx = ceil_div(N_, cover);
   // data end: only tiles that contain real points (the rest of Mpad is padding)
@@ -620,7 +672,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
   const int64_t mend = mtiles * kTM;
   int S = cfg_.multT_data_segs;
   if (S <= 0) {
-    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : 8 * num_cus_;
+    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : kBlocksPerCU * num_cus_;
     S = std::max(1, ceil_div(target, bx));
     S = static_cast<int>(std::min<int64_t>(S, std::max<int64_t>(1, mtiles / 4)));
   }
@@ -633,10 +685,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
     MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_));
-    // multTranspose evaluates every dimension: a wave's points span K tiles whose
-    // union of non-constant dimensions is close to all of them, and the
-    // per-dimension branches cost more than the factors they would skip
-    f(false, X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI,
+    f(skip_, X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI,
       static_cast<const uint32_t*>(tmask_), P, static_cast<int64_t>(Npad_), seg, mend, S,
       static_cast<int64_t>(N_), s);
   } else {
