@@ -20,10 +20,11 @@ constexpr int kMaxD = 20;
 constexpr int kMaxModels = 4096;
 
 struct ModelDesc {
-  const void* LI;  // [Npad][D][2] (L, I)
-  const void* A;   // [Npad] alpha * scale, zero padded
+  const void* LI;         // tile-major (L, I): [Npad/64][D][64][2]
+  const void* A;          // [Npad] alpha * scale, zero padded
+  const uint32_t* tmask;  // [Npad/64] dimensions evaluated per tile
   int D;
-  int Npad;        // multiple of kTG
+  int Npad;               // multiple of kTG
 };
 
 struct Item {
@@ -51,6 +52,14 @@ template <typename T>
 constexpr int serve_k(int D) {
   return sizeof(T) == 8 ? (D <= 5 ? 4 : (D <= 10 ? 2 : 1)) : (D <= 4 ? 8 : (D <= 10 ? 4 : 2));
 }
+// grid points per sub-tile (one wave-uniform branch per sub-tile and
+// dimension); J*K running products per lane, capped so that every dimension
+// case of the persistent kernel stays within the same register budget
+template <typename T>
+constexpr int serve_j(int D) {
+  return (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D) < 8 ? (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D)
+                                                          : 8;
+}
 template <typename T>
 int serve_k_rt(int D) {
   return sizeof(T) == 8 ? (D <= 5 ? 4 : (D <= 10 ? 2 : 1)) : (D <= 4 ? 8 : (D <= 10 ? 4 : 2));
@@ -63,8 +72,12 @@ __device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
 __device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
 __device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
 
-// one work item: a tile of kBlock*K queries of one request against one grid chunk
-template <int D, int K, typename T>
+// one work item: a tile of kBlock*K queries of one request against one grid
+// chunk. The grid tile is walked in sub-tiles of J points; per (sub-tile,
+// dimension) one wave-uniform test of the tile's mask decides whether the
+// dimension is evaluated (ModLinear level-1 factors are exactly 1; Linear
+// models carry full masks).
+template <int D, int K, int J, typename T>
 __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md, const T* X,
                                            double* part, T* sLI, T* sA) {
   const T* LI = static_cast<const T*>(md.LI);
@@ -80,6 +93,7 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
   for (int j0 = item.j0; j0 < item.j1; j0 += kTG) {
+    const uint32_t mask = md.tmask[j0 / kTG];
     __syncthreads();
     for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
       sLI[e] = LI[static_cast<long long>(j0) * D * 2 + e];
@@ -88,21 +102,32 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
-    for (int j = 0; j < kTG; ++j) {
-      T p[K];
+    for (int j = 0; j < kTG; j += J) {
+      T p[J][K];
+#pragma unroll
+      for (int q = 0; q < J; ++q)
+#pragma unroll
+        for (int k = 0; k < K; ++k) p[q][k] = T(1);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const T L = sLI[(j * D + d) * 2];
-        const T I = sLI[(j * D + d) * 2 + 1];
+        if (!((mask >> d) & 1u)) continue;
+        T li[2 * J];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const T u = fmaT(L, x[k][d], -I);
-          p[k] = (d == 0) ? clamp01(T(1) - absT(u)) : clamp01(fmaT(-p[k], absT(u), p[k]));
-        }
+        for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
+#pragma unroll
+        for (int q = 0; q < J; ++q)
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            const T u = fmaT(li[2 * q], x[k][d], -li[2 * q + 1]);
+            p[q][k] = clamp01(fmaT(-p[q][k], absT(u), p[q][k]));
+          }
       }
-      const T a = sA[j];
 #pragma unroll
-      for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[k], acc[k]);
+      for (int q = 0; q < J; ++q) {
+        const T a = sA[j + q];
+#pragma unroll
+        for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[q][k], acc[k]);
+      }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
@@ -115,7 +140,7 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 template <typename T, int D>
 __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md, const T* X,
                                              double* part, T* sLI, T* sA) {
-  serve_item<D, serve_k<T>(D), T>(it, md, X, part, sLI, sA);
+  serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sLI, sA);
 }
 
 // Persistent work-list kernel: blocks pull items from a counter until the
@@ -189,6 +214,7 @@ struct Server::Model {
   size_t N = 0, Npad = 0;
   void* LI = nullptr;
   void* A = nullptr;
+  void* tmask = nullptr;
   size_t bytes = 0;
   int tile = 0;    // queries per work item
   int chunk = 0;   // grid points per work item
@@ -334,6 +360,7 @@ Server::~Server() {
   for (auto& kv : models_) {
     if (kv.second->LI) (void)hipFree(kv.second->LI);
     if (kv.second->A) (void)hipFree(kv.second->A);
+    if (kv.second->tmask) (void)hipFree(kv.second->tmask);
   }
   if (model_table_) (void)hipFree(model_table_);
 }
@@ -342,8 +369,30 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
                         const uint32_t* index, const double* alpha) {
   if (dim < 1 || dim > static_cast<size_t>(kMaxD)) throw input_error("model dimension in [1, 20]");
   if (N == 0) throw input_error("empty model");
-  std::vector<double> li, sc;
-  encode_grid(basis, dim, N, level, index, li, sc);
+  // ModLinear: points sorted by their set of non-constant dimensions (level
+  // above 1), so that tiles share masks; the model's output is a sum over all
+  // points, so the order changes nothing but the summation order
+  const bool skip = basis == Basis::ModLinear && dim >= 2;
+  std::vector<uint32_t> pmask(N);
+  for (size_t j = 0; j < N; ++j) {
+    uint32_t mk = 0;
+    for (size_t d = 0; d < dim; ++d)
+      if (!skip || level[j * dim + d] > 1) mk |= 1u << d;
+    pmask[j] = mk;
+  }
+  std::vector<size_t> order(N);
+  for (size_t j = 0; j < N; ++j) order[j] = j;
+  if (skip)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return pmask[a] < pmask[b]; });
+  std::vector<uint32_t> lv(N * dim), ix(N * dim);
+  for (size_t s2 = 0; s2 < N; ++s2)
+    for (size_t d = 0; d < dim; ++d) {
+      lv[s2 * dim + d] = level[order[s2] * dim + d];
+      ix[s2 * dim + d] = index[order[s2] * dim + d];
+    }
+  std::vector<double> lip, sc;
+  encode_grid(basis, dim, N, lv.data(), ix.data(), lip, sc);
   auto m = std::make_shared<Model>();
   m->basis = basis;
   m->D = static_cast<int>(dim);
@@ -351,7 +400,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   m->Npad = ((N + kTG - 1) / kTG) * kTG;
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
-  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz;
+  const size_t ntiles = m->Npad / kTG;
+  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + ntiles * sizeof(uint32_t);
   const int K = f64 ? serve_k_rt<double>(m->D) : serve_k_rt<float>(m->D);
   m->tile = kBlock * K;
   const double per_point = static_cast<double>(m->tile) * (2 * dim + 1);
@@ -359,8 +409,17 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   chunk = std::max<size_t>(kTG, std::min(chunk, m->Npad));
   m->chunk = static_cast<int>(chunk);
   std::vector<double> A(m->Npad, 0.0);
-  for (size_t j = 0; j < N; ++j) A[j] = alpha[j] * sc[j];
-  li.resize(m->Npad * dim * 2, 0.0);
+  for (size_t s2 = 0; s2 < N; ++s2) A[s2] = alpha[order[s2]] * sc[s2];
+  // tile-major (L, I); padded points (0, 0) evaluate to 1 with weight 0
+  std::vector<double> li(m->Npad * dim * 2, 0.0);
+  for (size_t j = 0; j < N; ++j)
+    for (size_t d = 0; d < dim; ++d) {
+      const size_t dst = (((j / kTG) * dim + d) * kTG + (j % kTG)) * 2;
+      li[dst] = lip[(j * dim + d) * 2];
+      li[dst + 1] = lip[(j * dim + d) * 2 + 1];
+    }
+  std::vector<uint32_t> tm(ntiles, 0u);
+  for (size_t s2 = 0; s2 < N; ++s2) tm[s2 / kTG] |= pmask[order[s2]];
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (used_bytes_ + m->bytes > cfg_.memory_budget)
@@ -386,6 +445,9 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
     check_hip(hipMalloc(&m->LI, m->Npad * dim * 2 * esz), "hipMalloc(model LI)");
     check_hip(hipMalloc(&m->A, m->Npad * esz), "hipMalloc(model A)");
+    check_hip(hipMalloc(&m->tmask, ntiles * sizeof(uint32_t)), "hipMalloc(model masks)");
+    check_hip(hipMemcpy(m->tmask, tm.data(), ntiles * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "upload masks");
     if (f64) {
       check_hip(hipMemcpy(m->LI, li.data(), li.size() * 8, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, A.data(), A.size() * 8, hipMemcpyHostToDevice), "upload A");
@@ -394,7 +456,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       check_hip(hipMemcpy(m->LI, l32.data(), l32.size() * 4, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, a32.data(), a32.size() * 4, hipMemcpyHostToDevice), "upload A");
     }
-    ModelDesc md{m->LI, m->A, m->D, static_cast<int>(m->Npad)};
+    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->tmask), m->D,
+                 static_cast<int>(m->Npad)};
     check_hip(hipMemcpy(static_cast<ModelDesc*>(model_table_) + m->table_index, &md,
                         sizeof(ModelDesc), hipMemcpyHostToDevice),
               "upload model descriptor");
@@ -404,6 +467,7 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     used_bytes_ -= m->bytes;
     if (m->LI) (void)hipFree(m->LI);
     if (m->A) (void)hipFree(m->A);
+    if (m->tmask) (void)hipFree(m->tmask);
     throw;
   }
   return m->id;
@@ -422,6 +486,7 @@ void Server::evict_model(int id) {
   }
   check_hip(hipFree(m->LI), "hipFree");
   check_hip(hipFree(m->A), "hipFree");
+  check_hip(hipFree(m->tmask), "hipFree");
 }
 
 size_t Server::model_bytes(int id) {
