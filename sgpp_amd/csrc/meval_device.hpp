@@ -118,10 +118,12 @@ class DeviceMultiEval {
 
   // ModLinear grids are stored sorted by each point's set of non-constant
   // dimensions: perm_[stored position] = caller's index; tmask_[tile] = union
-  // of those sets over each 64-point tile
+  // of those sets over each 64-point tile, qmask_[q] the same over each 4
+  // consecutive points
   bool skip_ = false;
   void* perm_ = nullptr;   size_t perm_cap_ = 0;
   void* tmask_ = nullptr;  size_t tmask_cap_ = 0;
+  void* qmask_ = nullptr;  size_t qmask_cap_ = 0;
 
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;

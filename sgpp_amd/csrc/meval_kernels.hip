@@ -80,17 +80,19 @@ constexpr int mult_k(int D) {
   return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 5 ? 8 : (D <= 10 ? 4 : 2));
 }
 template <typename T>
-constexpr int mult_j() {
-  return sizeof(T) == 8 ? 4 : 8;
+constexpr int mult_j(int D) {
+  return sizeof(T) == 8 ? 4 : (mult_k<T>(D) == 8 ? 4 : 8);
 }
 // multTranspose: with the dimension skip one tile per wave (K = 1) keeps the
-// wave's mask as small as the tile's; without it two points per lane and
-// sub-tiles of 4 data points run fastest.
+// wave's mask as small as the tile's; without it 2-4 points per lane.
 template <typename T>
 constexpr int multT_k(int D, bool skip) {
-  return skip ? 1 : ((sizeof(T) == 8 && D > 10) ? 1 : 2);
+  return skip ? 1 : (sizeof(T) == 8 ? (D <= 10 ? 2 : 1) : (D <= 5 ? 4 : 2));
 }
-constexpr int multT_j(bool skip) { return skip ? 8 : 4; }
+template <typename T>
+constexpr int multT_j(int D, bool skip) {
+  return skip ? 8 : (sizeof(T) == 8 ? (multT_k<T>(D, false) == 1 ? 8 : 4) : (D <= 10 ? 4 : 8));
+}
 
 // (L, I) of a 64-point tile t and dimension d: LI[((t*D + d)*64 + jj)*2 + {0,1}]
 // (tile-major, then dimension, then point), so the J points of a sub-tile are
@@ -101,19 +103,20 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) {
 
 // ---------------------------------------------------------------- mult
 // X: [D][ldx] SoA, LI: tile-major pairs (li_at), A: [Npad] (alpha*scale, zero
-// padded), tmask: [Npad/64] non-constant dimensions of each tile (kSkip only),
-// part: [G][ldx]. Block covers kBlock*K data points and grid chunk
-// [blockIdx.y*chunk, min(+chunk, jend_all)).
+// padded), qmask: [Npad/4] non-constant dimensions of every 4 consecutive
+// points (kSkip only; a sub-tile ORs its J/4 masks), part: [G][ldx]. Block
+// covers kBlock*K data points and grid chunk [blockIdx.y*chunk, min(+chunk, jend_all)).
 template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
                                                       const T* __restrict__ A,
-                                                      const uint32_t* __restrict__ tmask,
+                                                      const uint32_t* __restrict__ qmask,
                                                       double* __restrict__ part, int chunk,
                                                       int jend_all) {
-  static_assert(kTG % J == 0, "sub-tile must divide the tile");
+  static_assert(kTG % J == 0 && J % 4 == 0, "sub-tile must divide the tile, 4 must divide it");
   __shared__ T sLI[kTG * D * 2];
   __shared__ T sA[kTG];
+  __shared__ uint32_t sM[kTG / 4];
   const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + threadIdx.x;
   T x[K][D];
 #pragma unroll
@@ -128,16 +131,23 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
   const int jb = blockIdx.y * chunk;
   const int je = min(jb + chunk, jend_all);
   for (int j0 = jb; j0 < je; j0 += kTG) {
-    const uint32_t mask = kSkip ? tmask[j0 / kTG] : 0xffffffffu;
     __syncthreads();
     for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
       sLI[e] = LI[static_cast<int64_t>(j0) * D * 2 + e];
     if (threadIdx.x < kTG) sA[threadIdx.x] = A[j0 + threadIdx.x];
+    if (kSkip && threadIdx.x < kTG / 4) sM[threadIdx.x] = qmask[j0 / 4 + threadIdx.x];
     __syncthreads();
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
     for (int j = 0; j < kTG; j += J) {
+      uint32_t mask = 0xffffffffu;
+      if (kSkip) {
+        mask = 0u;
+#pragma unroll
+        for (int c = 0; c < J / 4; ++c) mask |= sM[j / 4 + c];
+        mask = __builtin_amdgcn_readfirstlane(mask);
+      }
       T p[J][K];
 #pragma unroll
       for (int q = 0; q < J; ++q)
@@ -347,7 +357,7 @@ void launch_mult_D(bool skip, const T* X, int64_t ldx, const T* LI, const T* A,
                    const uint32_t* tm, double* part, int chunk, int jend, int G, int64_t Mpad,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
-  constexpr int J = mult_j<T>();
+  constexpr int J = mult_j<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
   if constexpr (D >= 2) {
@@ -372,7 +382,7 @@ void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
       constexpr int K = multT_k<T>(D, true);
       const int64_t cover = static_cast<int64_t>(kBlock) * K;
       dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-      hipLaunchKernelGGL((multT_kernel<D, K, multT_j(true), T, true>), grid, dim3(kBlock), 0, s,
+      hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D, true), T, true>), grid, dim3(kBlock), 0, s,
                          X, ldx, Y, LI, tm, part, ldp, seg, mend);
       return;
     }
@@ -380,7 +390,7 @@ void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
   constexpr int K = multT_k<T>(D, false);
   const int64_t cover = static_cast<int64_t>(kBlock) * K;
   dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-  hipLaunchKernelGGL((multT_kernel<D, K, multT_j(false), T, false>), grid, dim3(kBlock), 0, s, X,
+  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D, false), T, false>), grid, dim3(kBlock), 0, s, X,
                      ldx, Y, LI, tm, part, ldp, seg, mend);
 }
 
@@ -504,7 +514,8 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, scale_, perm_, tmask_, A_, Y_, part_, hin_, hout_};
+  void* bufs[] = {X64_, X32_, LI64_, LI32_, scale_, perm_, tmask_, qmask_, A_, Y_, part_, hin_,
+                  hout_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto* pool : {&tm_mult_, &tm_multT_})
@@ -583,6 +594,11 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
   ensure(&tmask_, &tmask_cap_, tm.size() * sizeof(uint32_t));
   check_hip(hipMemcpy(tmask_, tm.data(), tm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
             "upload tile masks");
+  std::vector<uint32_t> qm(Npad / 4, 0u);
+  for (size_t s = 0; s < N; ++s) qm[s / 4] |= pmask[order[s]];
+  ensure(&qmask_, &qmask_cap_, qm.size() * sizeof(uint32_t));
+  check_hip(hipMemcpy(qmask_, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+            "upload sub-tile masks");
   if (skip_) {
     ensure(&perm_, &perm_cap_, N_ * sizeof(int32_t));
     check_hip(hipMemcpy(perm_, order.data(), N_ * sizeof(int32_t), hipMemcpyHostToDevice),
@@ -637,7 +653,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
     MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_));
     f(skip_, X, static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_),
-      static_cast<const uint32_t*>(tmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
+      static_cast<const uint32_t*>(qmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();

@@ -22,7 +22,7 @@ constexpr int kMaxModels = 4096;
 struct ModelDesc {
   const void* LI;         // tile-major (L, I): [Npad/64][D][64][2]
   const void* A;          // [Npad] alpha * scale, zero padded
-  const uint32_t* tmask;  // [Npad/64] dimensions evaluated per tile
+  const uint32_t* qmask;  // [Npad/4] dimensions evaluated per 4 consecutive points
   int D;
   int Npad;               // multiple of kTG
 };
@@ -74,12 +74,13 @@ __device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__bu
 
 // one work item: a tile of kBlock*K queries of one request against one grid
 // chunk. The grid tile is walked in sub-tiles of J points; per (sub-tile,
-// dimension) one wave-uniform test of the tile's mask decides whether the
+// dimension) one wave-uniform test of the sub-tile's mask decides whether the
 // dimension is evaluated (ModLinear level-1 factors are exactly 1; Linear
 // models carry full masks).
 template <int D, int K, int J, typename T>
 __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md, const T* X,
-                                           double* part, T* sLI, T* sA) {
+                                           double* part, T* sLI, T* sA, uint32_t* sM) {
+  static_assert(kTG % J == 0 && J % 4 == 0, "sub-tile must divide the tile, 4 must divide it");
   const T* LI = static_cast<const T*>(md.LI);
   const T* A = static_cast<const T*>(md.A);
   const T* Xr = X + item.xoff;
@@ -93,16 +94,20 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
   for (int j0 = item.j0; j0 < item.j1; j0 += kTG) {
-    const uint32_t mask = md.tmask[j0 / kTG];
     __syncthreads();
     for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
       sLI[e] = LI[static_cast<long long>(j0) * D * 2 + e];
     if (threadIdx.x < kTG) sA[threadIdx.x] = A[j0 + threadIdx.x];
+    if (threadIdx.x < kTG / 4) sM[threadIdx.x] = md.qmask[j0 / 4 + threadIdx.x];
     __syncthreads();
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
     for (int j = 0; j < kTG; j += J) {
+      uint32_t mask = 0u;
+#pragma unroll
+      for (int c = 0; c < J / 4; ++c) mask |= sM[j / 4 + c];
+      mask = __builtin_amdgcn_readfirstlane(mask);
       T p[J][K];
 #pragma unroll
       for (int q = 0; q < J; ++q)
@@ -139,8 +144,8 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 
 template <typename T, int D>
 __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md, const T* X,
-                                             double* part, T* sLI, T* sA) {
-  serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sLI, sA);
+                                             double* part, T* sLI, T* sA, uint32_t* sM) {
+  serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sLI, sA, sM);
 }
 
 // Persistent work-list kernel: blocks pull items from a counter until the
@@ -153,6 +158,7 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restri
                                                        double* __restrict__ part) {
   __shared__ T sLI[kTG * kMaxD * 2];
   __shared__ T sA[kTG];
+  __shared__ uint32_t sM[kTG / 4];
   __shared__ unsigned s_item;
   const unsigned n = static_cast<unsigned>(counts[0]);
   for (;;) {
@@ -166,7 +172,7 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restri
     switch (item.D) {
 #define SGPP_CASE(DD) \
   case DD:            \
-    dispatch_one<T, DD>(item, md, X, part, sLI, sA); \
+    dispatch_one<T, DD>(item, md, X, part, sLI, sA, sM); \
     break;
       SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
       SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
@@ -400,8 +406,7 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   m->Npad = ((N + kTG - 1) / kTG) * kTG;
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
-  const size_t ntiles = m->Npad / kTG;
-  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + ntiles * sizeof(uint32_t);
+  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + (m->Npad / 4) * sizeof(uint32_t);
   const int K = f64 ? serve_k_rt<double>(m->D) : serve_k_rt<float>(m->D);
   m->tile = kBlock * K;
   const double per_point = static_cast<double>(m->tile) * (2 * dim + 1);
@@ -418,8 +423,9 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       li[dst] = lip[(j * dim + d) * 2];
       li[dst + 1] = lip[(j * dim + d) * 2 + 1];
     }
-  std::vector<uint32_t> tm(ntiles, 0u);
-  for (size_t s2 = 0; s2 < N; ++s2) tm[s2 / kTG] |= pmask[order[s2]];
+  // padded points keep mask 0: nothing evaluated, weight 0
+  std::vector<uint32_t> qm(m->Npad / 4, 0u);
+  for (size_t s2 = 0; s2 < N; ++s2) qm[s2 / 4] |= pmask[order[s2]];
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (used_bytes_ + m->bytes > cfg_.memory_budget)
@@ -445,8 +451,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
     check_hip(hipMalloc(&m->LI, m->Npad * dim * 2 * esz), "hipMalloc(model LI)");
     check_hip(hipMalloc(&m->A, m->Npad * esz), "hipMalloc(model A)");
-    check_hip(hipMalloc(&m->tmask, ntiles * sizeof(uint32_t)), "hipMalloc(model masks)");
-    check_hip(hipMemcpy(m->tmask, tm.data(), ntiles * sizeof(uint32_t), hipMemcpyHostToDevice),
+    check_hip(hipMalloc(&m->tmask, qm.size() * sizeof(uint32_t)), "hipMalloc(model masks)");
+    check_hip(hipMemcpy(m->tmask, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
               "upload masks");
     if (f64) {
       check_hip(hipMemcpy(m->LI, li.data(), li.size() * 8, hipMemcpyHostToDevice), "upload LI");

@@ -53,6 +53,33 @@ def test_parity_regular(gpu, basis, prec, d, level, M):
     assert rel(a, a0) <= TOL[prec], rel(a, a0)
 
 
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("d,M", [(2, 3001), (3, 2500), (7, 2000), (12, 1500), (20, 800)])
+def test_parity_adaptive(gpu, basis, prec, d, M):
+    """Adaptive grids (regular level 2 or 3, then surplus refinement driven by
+    a random indicator) whose points have varied sets of non-constant
+    dimensions: exercises the ModLinear storage order, the per-tile and
+    per-4-point masks and partly filled tiles."""
+    capi = gpu
+    g = capi.Grid(basis, d).regular(3 if d <= 7 else 2)
+    for step in range(4 if d <= 7 else 3):
+        g.refine_surplus(1.0 + workloads.uniform(workloads.stream(200 + d, step), g.size),
+                         max(1, g.size // 6))
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(300 + d, M, d)
+    if prec == 1:
+        X = X.astype(np.float32).astype(np.float64)
+    alpha = workloads.normal(workloads.stream(300 + d, 1), g.size)
+    y = workloads.normal(workloads.stream(300 + d, 2), M)
+    print("case basis=%d prec=%d d=%d M=%d N=%d" % (basis, prec, d, M, g.size), flush=True)
+    op = capi.MultiEval(basis, X, precision=prec).prepare(lv, ix)
+    r = op.mult(alpha)
+    a = op.mult_transpose(y)
+    assert rel(r, oracle.mult(basis, X, lv, ix, alpha)) <= TOL[prec]
+    assert rel(a, oracle.mult_transpose(basis, X, lv, ix, y)) <= TOL[prec]
+
+
 def test_small_linear_workload(gpu):
     """Linear d=5 level 5 (N=1471), 10,000 uniform points: FP64 <= 1e-12 against
     the definition and against the reference's StreamingDefault / SubspaceLinear."""
