@@ -59,6 +59,9 @@ struct LaunchConfig {
 void encode_grid(Basis basis, size_t dim, size_t n, const uint32_t* level, const uint32_t* index,
                  std::vector<double>& li, std::vector<double>& scale);
 
+// Morton order of n points in [0,1]^d (row-major); ties keep input order.
+std::vector<int32_t> morton_order(const double* pts, size_t n, size_t d);
+
 class DeviceMultiEval {
  public:
   // data_rowmajor: M x d, FP64 row-major (the caller's DataMatrix layout).
@@ -111,19 +114,20 @@ class DeviceMultiEval {
   void record_start();
   void record_stop();
   template <typename T>
-  void run_mult(const T* X, const T* LI, const double* d_alpha, double* d_result,
+  void run_mult(const T* X, const T* LI, const T* TB, const double* d_alpha, double* d_result,
                 hipEvent_t* tev);
   template <typename T>
-  void run_multT(const T* X, const T* LI, const double* d_y, double* d_result, hipEvent_t* tev);
+  void run_multT(const T* X, const T* LI, const T* TB, const double* d_y, double* d_result,
+                 hipEvent_t* tev);
 
-  // ModLinear grids are stored sorted by each point's set of non-constant
-  // dimensions: perm_[stored position] = caller's index; tmask_[tile] = union
-  // of those sets over each 64-point tile, qmask_[q] the same over each 4
-  // consecutive points
-  bool skip_ = false;
+  // Storage orders. Data: Morton order, dperm_[stored row] = caller's row.
+  // Grid: by each point's set of non-constant dimensions, then Morton order
+  // of the support centres; perm_[stored position] = caller's index,
+  // pmask_[stored position] = that set (all dimensions for Linear, 0 for
+  // padding).
+  void* dperm_ = nullptr;
   void* perm_ = nullptr;   size_t perm_cap_ = 0;
-  void* tmask_ = nullptr;  size_t tmask_cap_ = 0;
-  void* qmask_ = nullptr;  size_t qmask_cap_ = 0;
+  void* pmask_ = nullptr;  size_t pmask_cap_ = 0;
 
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;
@@ -134,14 +138,17 @@ class DeviceMultiEval {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
 
-  // data, SoA dim-major [D][Mpad]
+  // data in storage order, SoA dim-major [D][Mpad]
   void* X64_ = nullptr;
   void* X32_ = nullptr;
-  // grid pairs [Npad][D][2] and per-point scale [Npad]
+  // grid pairs, point-major [Npad][D][2], per-point scale [Npad] and
+  // per-tile (64 points) boxes of the supports [Npad/64][D][2]
   void* LI64_ = nullptr;
   void* LI32_ = nullptr;
+  void* TB64_ = nullptr;
+  void* TB32_ = nullptr;
   double* scale_ = nullptr;
-  size_t gridcap_li64_ = 0, gridcap_li32_ = 0, gridcap_scale_ = 0;
+  size_t gridcap_li64_ = 0, gridcap_li32_ = 0, gridcap_scale_ = 0, tbcap64_ = 0, tbcap32_ = 0;
 
   // scratch
   void* A_ = nullptr;     size_t A_cap_ = 0;      // alpha*scale in compute type
@@ -164,7 +171,7 @@ class DeviceMultiEval {
 };
 
 // Padding units (points): data is padded to a multiple of kDataPad by
-// repeating the last row, the grid to a multiple of kGridPad with
+// repeating the last stored row, the grid to a multiple of kGridPad with
 // points that evaluate to a constant and carry zero weight.
 constexpr size_t kDataPad = 4096;
 constexpr size_t kGridPad = 2048;

@@ -51,9 +51,19 @@ constexpr int kBlock = 256;  // 4 waves of 64
 constexpr int kTG = 64;      // grid points per LDS tile (mult) and per mask tile
 constexpr int kTM = 128;     // data points per LDS tile (multTranspose)
 constexpr int kMaxTemplD = 20;
-// default launch size: workgroups per CU and launch (measured optimum region
-// 40-300 on the d=10 grids; below ~16 the last wave of blocks leaves CUs idle)
-constexpr int kBlocksPerCU = 64;
+// Default launch sizes. Culled blocks cost very different amounts, so many
+// short blocks balance better: mult ~128 workgroups per CU and grid chunks of
+// at most 24 tiles; multTranspose ~256 workgroups per CU (each block covers
+// kChunkT grid points). Measured on the d=10 ModLinear (1e5 x 1e6) and d=8
+// Linear (3.2e4 x 5e5) workloads.
+constexpr int kBlocksPerCU = 128;
+constexpr int kMaxChunkTiles = 24;
+constexpr int kBlocksPerCUT = 256;
+// cap on the FP64 partials of one launch (G x Mpad or S x Npad doubles)
+constexpr int64_t kPartialBudget = int64_t(512) << 20;
+// data rows past M kept in the padded data (a wave's last group of 64*K
+// points may start just below M)
+constexpr size_t kDataSlack = 1024;
 
 __device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -69,85 +79,243 @@ __device__ __forceinline__ T hat_step(T p, T L, T x, T I) {
   return clamp01(fmaT(-p, absT(u), p));
 }
 
-// Points per lane (K) and points per sub-tile (J). One wave-uniform branch
-// per (sub-tile, dimension) decides whether a dimension is evaluated, so the
-// branch cost is shared by J*K factor updates per lane. K keeps the per-lane
-// operands (coordinates for mult, (L, I) for multTranspose) within ~40 FP32
-// or ~80 FP64 registers. Values from launch-shape and variant sweeps on the
-// d=10 ModLinear headline grid.
+// Points per lane (K) and grid points evaluated together (J). Values from
+// variant sweeps on the d=10 ModLinear and d=8 Linear workloads.
 template <typename T>
 constexpr int mult_k(int D) {
-  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 5 ? 8 : (D <= 10 ? 4 : 2));
+  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
 }
 template <typename T>
 constexpr int mult_j(int D) {
-  return sizeof(T) == 8 ? 4 : (mult_k<T>(D) == 8 ? 4 : 8);
-}
-// multTranspose: with the dimension skip one tile per wave (K = 1) keeps the
-// wave's mask as small as the tile's; without it 2-4 points per lane.
-template <typename T>
-constexpr int multT_k(int D, bool skip) {
-  return skip ? 1 : (sizeof(T) == 8 ? (D <= 10 ? 2 : 1) : (D <= 5 ? 4 : 2));
+  return 2;
 }
 template <typename T>
-constexpr int multT_j(int D, bool skip) {
-  return skip ? 8 : (sizeof(T) == 8 ? (multT_k<T>(D, false) == 1 ? 8 : 4) : (D <= 10 ? 4 : 8));
+constexpr int multT_k(int D) {
+  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
+}
+template <typename T>
+constexpr int multT_j(int D) {
+  return 2;
+}
+// grid points per multTranspose block (FP64 accumulators in LDS, one row per wave)
+constexpr int kChunkT = 1024;
+
+// (L, I) of grid point j and dimension d: LI[(j*D + d)*2 + {0,1}] (point-major:
+// one point's 2*D values are contiguous, so a wave fetches a taken point's
+// whole row with one wide scalar load; tile t = points [64t, 64t+64)).
+__device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * D + d) * 2; }
+
+// ------------------------------------------------------- wave helpers
+// Cross-lane moves with DPP row controls (VALU operand modifiers, no LDS
+// traffic). Lanes whose source is outside the row, or whose row is not in
+// RowMask, receive `old`.
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ float dpp(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), Ctrl, RowMask, 0xf, false));
+}
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ double dpp(double old, double v) {
+  const long long o = __double_as_longlong(old), s = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), Ctrl, RowMask,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32),
+                                             Ctrl, RowMask, 0xf, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// Reduction over the 64 lanes; the result is complete in lane 63 only.
+// Steps: swap within pairs and halves of each quad (every lane then holds its
+// quad's value), shift right by 4 and 8 within each row of 16 (lane 15 of a
+// row holds the row's value), broadcast lane 15 into the next row (lanes 31
+// and 63 hold two rows each), broadcast lane 31 into rows 2-3 (lane 63).
+// For sums the lanes outside a step's reach add 0; for min/max they combine
+// with themselves.
+template <typename T>
+__device__ __forceinline__ T sum_to_lane63(T v) {
+  v += dpp<0xb1>(T(0), v);        // quad_perm [1,0,3,2]
+  v += dpp<0x4e>(T(0), v);        // quad_perm [2,3,0,1]
+  v += dpp<0x114>(T(0), v);       // row_shr:4
+  v += dpp<0x118>(T(0), v);       // row_shr:8
+  v += dpp<0x142, 0xa>(T(0), v);  // row_bcast:15 into rows 1, 3
+  v += dpp<0x143, 0xc>(T(0), v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T min_to_lane63(T v) {
+  v = fmin(v, dpp<0xb1>(v, v));
+  v = fmin(v, dpp<0x4e>(v, v));
+  v = fmin(v, dpp<0x114>(v, v));
+  v = fmin(v, dpp<0x118>(v, v));
+  v = fmin(v, dpp<0x142, 0xa>(v, v));
+  v = fmin(v, dpp<0x143, 0xc>(v, v));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T max_to_lane63(T v) {
+  v = fmax(v, dpp<0xb1>(v, v));
+  v = fmax(v, dpp<0x4e>(v, v));
+  v = fmax(v, dpp<0x114>(v, v));
+  v = fmax(v, dpp<0x118>(v, v));
+  v = fmax(v, dpp<0x142, 0xa>(v, v));
+  v = fmax(v, dpp<0x143, 0xc>(v, v));
+  return v;
+}
+// lane 63's value, moved to a scalar register
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// Box [lo, hi] per dimension of the wave's 64*K data points.
+template <int D, int K, typename T>
+__device__ __forceinline__ void wave_box(const T (&x)[K][D], T (&lo)[D], T (&hi)[D]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    T a = x[0][d], b = x[0][d];
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      a = fmin(a, x[k][d]);
+      b = fmax(b, x[k][d]);
+    }
+    lo[d] = lane63(min_to_lane63(a));
+    hi[d] = lane63(max_to_lane63(b));
+  }
 }
 
-// (L, I) of a 64-point tile t and dimension d: LI[((t*D + d)*64 + jj)*2 + {0,1}]
-// (tile-major, then dimension, then point), so the J points of a sub-tile are
-// contiguous for every dimension and a wave's 64 points are contiguous too.
-__device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) {
-  return (((j >> 6) * D + d) * 64 + (j & 63)) * 2;
+// False only if the factor max(0, 1 - |L x - I|) (L >= 0) is exactly zero for
+// every x in [lo, hi]: in exact arithmetic that holds when L*hi - I <= -1 or
+// L*lo - I >= 1. fma rounds monotonically, so "fma(L, hi, -I) < -1" implies
+// L*hi - I < -1: a point is only ever dropped when the kernel's own
+// u = fma(L, x, -I) has |u| >= 1 for all x in the box.
+template <typename T>
+__device__ __forceinline__ bool may_hit(T L, T I, T lo, T hi) {
+  return !(fmaT(L, hi, -I) < T(-1)) && !(fmaT(L, lo, -I) > T(1));
+}
+
+// Tile pre-test: the tile's union of supports TB[t][d] = [lo, hi] (rounded
+// outwards) against the wave's data box. All 2*D bounds are loaded before any
+// is tested (no short-circuit), so the loads are issued together.
+template <int D, typename T>
+__device__ __forceinline__ bool tile_may_hit(const T* __restrict__ TB, int64_t t, const T (&lo)[D],
+                                             const T (&hi)[D]) {
+  const T* tb = TB + t * D * 2;
+  T b[2 * D];
+#pragma unroll
+  for (int e = 0; e < 2 * D; ++e) b[e] = tb[e];
+  unsigned h = 1u;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    h &= static_cast<unsigned>(b[2 * d] < hi[d]) & static_cast<unsigned>(b[2 * d + 1] > lo[d]);
+  return h != 0u;
+}
+
+// One lane per point of the tile: which points may be nonzero in the box
+// (the lane's whole (L, I) row is loaded before the tests).
+template <int D, typename T>
+__device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ tile, int lane,
+                                                        const T (&lo)[D], const T (&hi)[D]) {
+  const T* row = tile + lane * D * 2;
+  T r[2 * D];
+#pragma unroll
+  for (int e = 0; e < 2 * D; ++e) r[e] = row[e];
+  unsigned h = 1u;
+#pragma unroll
+  for (int d = 0; d < D; ++d) h &= static_cast<unsigned>(may_hit(r[2 * d], r[2 * d + 1], lo[d], hi[d]));
+  return __ballot(h != 0u);
+}
+
+// The (L, I) rows of the J taken points of a tile, loaded together
+// (wave-uniform addresses: scalar loads).
+template <int D, int J, typename T>
+__device__ __forceinline__ void load_rows(const T* __restrict__ tile, const int (&jj)[J],
+                                          T (&li)[J][2 * D]) {
+#pragma unroll
+  for (int q = 0; q < J; ++q)
+#pragma unroll
+    for (int e = 0; e < 2 * D; ++e) li[q][e] = tile[jj[q] * D * 2 + e];
+}
+
+// Takes up to J points from the hit mask (wave-uniform). Missing slots
+// repeat slot 0 and are flagged unused; mask = union of the taken points'
+// non-constant dimensions; w = the points' weights W[j0 + jj] (0 for unused
+// slots; none without kW), loaded before the products so that their latency
+// overlaps the evaluation.
+template <int J, bool kW, typename T>
+__device__ __forceinline__ void take_hits(unsigned long long& bal, const uint32_t* __restrict__ pm,
+                                          const T* __restrict__ W, int j0, int (&jj)[J],
+                                          bool (&used)[J], T (&w)[J], uint32_t& mask) {
+  mask = 0u;
+#pragma unroll
+  for (int q = 0; q < J; ++q) {
+    if (bal) {
+      const int b = __builtin_ctzll(bal);
+      bal &= bal - 1;
+      jj[q] = b;
+      used[q] = true;
+      mask |= pm[j0 + b];
+      w[q] = kW ? W[j0 + b] : T(0);
+    } else {
+      jj[q] = jj[0];
+      used[q] = false;
+      w[q] = T(0);
+    }
+  }
 }
 
 // ---------------------------------------------------------------- mult
-// X: [D][ldx] SoA, LI: tile-major pairs (li_at), A: [Npad] (alpha*scale, zero
-// padded), qmask: [Npad/4] non-constant dimensions of every 4 consecutive
-// points (kSkip only; a sub-tile ORs its J/4 masks), part: [G][ldx]. Block
-// covers kBlock*K data points and grid chunk [blockIdx.y*chunk, min(+chunk, jend_all)).
-template <int D, int K, int J, typename T, bool kSkip>
+// Lanes own data points (K per lane, coordinates in registers). The data are
+// stored in Morton order, so a wave's 64*K points fill a small box. For every
+// 64-point grid tile of its chunk a wave
+//   1. skips the tile if the tile's box of supports misses its data box;
+//   2. tests the tile's points against the box (one point per lane, ballot);
+//   3. evaluates only points that may be nonzero in the box, J at a time,
+//      over the union of their non-constant dimensions.
+// Every dropped term is exactly zero, so the sums equal a full evaluation.
+// X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] tile boxes,
+// A: [Npad] alpha*scale (zero padded), pm: [Npad] per-point dimension masks,
+// part: [G][ldx]. Block: kBlock*K data points x grid chunk [y*chunk, +chunk).
+template <int D, int K, int J, typename T>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
+                                                      const T* __restrict__ TB,
                                                       const T* __restrict__ A,
-                                                      const uint32_t* __restrict__ qmask,
+                                                      const uint32_t* __restrict__ pm,
                                                       double* __restrict__ part, int chunk,
                                                       int jend_all) {
-  static_assert(kTG % J == 0 && J % 4 == 0, "sub-tile must divide the tile, 4 must divide it");
-  __shared__ T sLI[kTG * D * 2];
-  __shared__ T sA[kTG];
-  __shared__ uint32_t sM[kTG / 4];
   const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + threadIdx.x;
+  const int lane = threadIdx.x & 63;
   T x[K][D];
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * kBlock];
-
+  T lo[D], hi[D];
+  wave_box<D, K, T>(x, lo, hi);
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
-
   const int jb = blockIdx.y * chunk;
   const int je = min(jb + chunk, jend_all);
   for (int j0 = jb; j0 < je; j0 += kTG) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
-      sLI[e] = LI[static_cast<int64_t>(j0) * D * 2 + e];
-    if (threadIdx.x < kTG) sA[threadIdx.x] = A[j0 + threadIdx.x];
-    if (kSkip && threadIdx.x < kTG / 4) sM[threadIdx.x] = qmask[j0 / 4 + threadIdx.x];
-    __syncthreads();
+    const int64_t t = j0 / kTG;
+    if (!tile_may_hit<D, T>(TB, t, lo, hi)) continue;
+    const T* tile = LI + t * D * kTG * 2;
+    unsigned long long bal = tile_hits<D, T>(tile, lane, lo, hi);
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
-    for (int j = 0; j < kTG; j += J) {
-      uint32_t mask = 0xffffffffu;
-      if (kSkip) {
-        mask = 0u;
-#pragma unroll
-        for (int c = 0; c < J / 4; ++c) mask |= sM[j / 4 + c];
-        mask = __builtin_amdgcn_readfirstlane(mask);
-      }
+    while (bal) {
+      int jj[J];
+      bool used[J];
+      T a[J];
+      uint32_t mask;
+      take_hits<J, true, T>(bal, pm, A, j0, jj, used, a, mask);
+      T li[J][2 * D];
+      load_rows<D, J, T>(tile, jj, li);
       T p[J][K];
 #pragma unroll
       for (int q = 0; q < J; ++q)
@@ -155,22 +323,16 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
         for (int k = 0; k < K; ++k) p[q][k] = T(1);
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        if (kSkip && !((mask >> d) & 1u)) continue;
-        // (L, I) of the sub-tile's J points in dimension d: one batch of LDS reads
-        T li[2 * J];
-#pragma unroll
-        for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
+        if (!((mask >> d) & 1u)) continue;
 #pragma unroll
         for (int q = 0; q < J; ++q)
 #pragma unroll
-          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[2 * q], x[k][d], li[2 * q + 1]);
+          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[q][2 * d], x[k][d], li[q][2 * d + 1]);
       }
 #pragma unroll
-      for (int q = 0; q < J; ++q) {
-        const T a = sA[j + q];
+      for (int q = 0; q < J; ++q)
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[q][k], acc[k]);
-      }
+        for (int k = 0; k < K; ++k) acc[k] = fmaT(a[q], p[q][k], acc[k]);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
@@ -181,7 +343,7 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
 }
 
 // Generic-dimension mult (D > kMaxTemplD): one data point per lane,
-// coordinates re-read from global memory (cached), runtime loops.
+// coordinates re-read from global memory (cached), runtime loops, no culling.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restrict__ X, int64_t ldx,
                                                               const T* __restrict__ LI,
@@ -208,85 +370,96 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 }
 
 // ------------------------------------------------------- multTranspose
-// X: [D][ldx] SoA, Y: [ldx] (zero padded), LI: tile-major pairs (li_at),
-// part: [S][ldp]. Block covers grid points [blockIdx.x*kBlock*K, +kBlock*K)
-// and data segment [blockIdx.y*seg, min(+seg, mend)). The data tile sits in
-// LDS as [D+1][kTM] (coordinates, then y). With kSkip each wave evaluates
-// only the union of the non-constant dimensions of its K tiles.
-template <int D, int K, int J, typename T, bool kSkip>
+// Same orientation as mult: lanes own data points (K per lane, coordinates
+// and y in registers), grid tiles are culled against the wave's data box and
+// the surviving points are evaluated J at a time. A point's contribution
+// sum_k y_k phi(x_k) is summed over the wave's lanes (butterfly) and added,
+// in FP64, to the wave's own row of LDS accumulators; at the end the four
+// rows are added in a fixed order. Block: grid chunk
+// [blockIdx.x*kChunkT, +kChunkT) x data segment [blockIdx.y*seg, +seg), the
+// segment split into four contiguous wave shares. part: [S][ldp].
+template <int D, int K, int J, typename T>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
-                                                       const uint32_t* __restrict__ tmask,
+                                                       const T* __restrict__ TB,
+                                                       const uint32_t* __restrict__ pm,
                                                        double* __restrict__ part, int64_t ldp,
-                                                       int64_t seg, int64_t mend) {
-  static_assert(kTM % J == 0, "sub-tile must divide the tile");
-  __shared__ T sX[(D + 1) * kTM];
-  const int64_t jblock = static_cast<int64_t>(blockIdx.x) * (kBlock * K);
-  const int64_t j0 = jblock + threadIdx.x;
-  uint32_t mask = 0xffffffffu;
-  if (kSkip) {
-    const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x) >> 6);
-    mask = 0u;
-#pragma unroll
-    for (int k = 0; k < K; ++k) mask |= tmask[(jblock + wave * 64 + k * kBlock) / kTG];
-  }
-  T L[K][D], I[K][D];
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int64_t e = li_at(j0 + k * kBlock, d, D);
-      L[k][d] = LI[e];
-      I[k][d] = LI[e + 1];
-    }
-  double accd[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) accd[k] = 0.0;
-
+                                                       int jend_all, int64_t seg, int64_t mend) {
+  __shared__ double accs[kBlock / 64][kChunkT];
+  const int wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const int jb = blockIdx.x * kChunkT;
+  const int je = min(jb + kChunkT, jend_all);
+  for (int i = threadIdx.x; i < (kBlock / 64) * kChunkT; i += kBlock) (&accs[0][0])[i] = 0.0;
+  __syncthreads();
   const int64_t mb = static_cast<int64_t>(blockIdx.y) * seg;
   const int64_t me = min(mb + seg, mend);
-  for (int64_t m0 = mb; m0 < me; m0 += kTM) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kTM * (D + 1); e += kBlock) {
-      const int d = e / kTM;
-      const int m = e - d * kTM;
-      sX[e] = (d < D) ? X[d * ldx + m0 + m] : Y[m0 + m];
+  // per-wave share, a multiple of 64*K; points past the share weigh 0
+  const int64_t share = ((me - mb) / (kBlock / 64) + 64 * K - 1) / (64 * K) * (64 * K);
+  const int64_t wb = mb + wave * share;
+  const int64_t we = min(wb + share, me);
+  double* acc = accs[wave];
+  for (int64_t m0 = wb; m0 < we; m0 += 64 * K) {
+    T x[K][D], y[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int64_t m = m0 + k * 64 + lane;
+#pragma unroll
+      for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m];
+      y[k] = (m < we) ? Y[m] : T(0);
     }
-    __syncthreads();
-    T acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = T(0);
-    for (int m = 0; m < kTM; m += J) {
-      T p[J][K];
-#pragma unroll
-      for (int q = 0; q < J; ++q)
-#pragma unroll
-        for (int k = 0; k < K; ++k) p[q][k] = T(1);
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (kSkip && !((mask >> d) & 1u)) continue;
-        T xv[J];
-#pragma unroll
-        for (int q = 0; q < J; ++q) xv[q] = sX[d * kTM + m + q];
+    T lo[D], hi[D];
+    wave_box<D, K, T>(x, lo, hi);
+    for (int j0 = jb; j0 < je; j0 += kTG) {
+      const int64_t t = j0 / kTG;
+      if (!tile_may_hit<D, T>(TB, t, lo, hi)) continue;
+      const T* tile = LI + t * D * kTG * 2;
+      unsigned long long bal = tile_hits<D, T>(tile, lane, lo, hi);
+      while (bal) {
+        int jj[J];
+        bool used[J];
+        T unused_w[J];
+        uint32_t mask;
+        take_hits<J, false, T>(bal, pm, static_cast<const T*>(nullptr), j0, jj, used, unused_w, mask);
+        T li[J][2 * D];
+        load_rows<D, J, T>(tile, jj, li);
+        T p[J][K];
 #pragma unroll
         for (int q = 0; q < J; ++q)
 #pragma unroll
-          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], L[k][d], xv[q], I[k][d]);
-      }
+          for (int k = 0; k < K; ++k) p[q][k] = T(1);
 #pragma unroll
-      for (int q = 0; q < J; ++q) {
-        const T y = sX[D * kTM + m + q];
+        for (int d = 0; d < D; ++d) {
+          if (!((mask >> d) & 1u)) continue;
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = fmaT(y, p[q][k], acc[k]);
+          for (int q = 0; q < J; ++q)
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+              p[q][k] = hat_step(p[q][k], li[q][2 * d], x[k][d], li[q][2 * d + 1]);
+        }
+#pragma unroll
+        for (int q = 0; q < J; ++q) {
+          if (!used[q]) continue;
+          T c = T(0);
+#pragma unroll
+          for (int k = 0; k < K; ++k) c = fmaT(y[k], p[q][k], c);
+          c = sum_to_lane63(c);
+          // one lane adds to the wave's own accumulator row (LDS atomics of
+          // one wave complete in issue order: deterministic)
+          if (lane == 63) atomicAdd(&acc[j0 - jb + jj[q]], static_cast<double>(c));
+        }
       }
     }
-#pragma unroll
-    for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
   }
+  __syncthreads();
   double* out = part + static_cast<int64_t>(blockIdx.y) * ldp;
+  for (int i = threadIdx.x; i < je - jb; i += kBlock) {
+    double s = accs[0][i];
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[j0 + k * kBlock] = accd[k];
+    for (int w = 1; w < kBlock / 64; ++w) s += accs[w][i];
+    out[jb + i] = s;
+  }
 }
 
 template <typename T>
@@ -331,11 +504,12 @@ __global__ void prep_alpha_kernel(const double* __restrict__ alpha, const double
   }
 }
 
+// Y[m] = y[dperm[m]] for stored position m (data in storage order), 0 past n
 template <typename T>
-__global__ void prep_y_kernel(const double* __restrict__ y, T* __restrict__ Y, int64_t n,
-                              int64_t npad) {
+__global__ void prep_y_kernel(const double* __restrict__ y, const int32_t* __restrict__ dperm,
+                              T* __restrict__ Y, int64_t n, int64_t npad) {
   const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m < npad) Y[m] = (m < n) ? static_cast<T>(y[m]) : T(0);
+  if (m < npad) Y[m] = (m < n) ? static_cast<T>(y[dperm[m]]) : T(0);
 }
 
 // out[perm ? perm[i] : i] = (scale ? scale[i] : 1) * sum_{p < nparts} part[p*ld + i],
@@ -353,53 +527,33 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 
 // ------------------------------------------------------------ dispatch
 template <typename T, int D>
-void launch_mult_D(bool skip, const T* X, int64_t ldx, const T* LI, const T* A,
-                   const uint32_t* tm, double* part, int chunk, int jend, int G, int64_t Mpad,
+void launch_mult_D(const T* X, int64_t ldx, const T* LI, const T* TB, const T* A,
+                   const uint32_t* pm, double* part, int chunk, int jend, int G, int64_t Mpad,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
-  constexpr int J = mult_j<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
-  if constexpr (D >= 2) {
-    if (skip) {
-      hipLaunchKernelGGL((mult_kernel<D, K, J, T, true>), grid, dim3(kBlock), 0, s, X, ldx, LI, A,
-                         tm, part, chunk, jend);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((mult_kernel<D, K, J, T, false>), grid, dim3(kBlock), 0, s, X, ldx, LI, A,
-                     tm, part, chunk, jend);
+  hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, LI,
+                     TB, A, pm, part, chunk, jend);
 }
 
 template <typename T, int D>
-void launch_multT_D(bool skip, const T* X, int64_t ldx, const T* Y, const T* LI,
-                    const uint32_t* tm, double* part, int64_t ldp, int64_t seg, int64_t mend,
-                    int S, int64_t N, hipStream_t s) {
+void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const T* TB,
+                    const uint32_t* pm, double* part, int64_t ldp, int jend, int64_t seg,
+                    int64_t mend, int S, hipStream_t s) {
+  constexpr int K = multT_k<T>(D);
+  dim3 grid(static_cast<unsigned>((jend + kChunkT - 1) / kChunkT), static_cast<unsigned>(S));
   clear_launch_error();
-  // one dimension leaves nothing to skip (and its skip variant compiles badly)
-  if constexpr (D >= 2) {
-    if (skip) {
-      constexpr int K = multT_k<T>(D, true);
-      const int64_t cover = static_cast<int64_t>(kBlock) * K;
-      dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-      hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D, true), T, true>), grid, dim3(kBlock), 0, s,
-                         X, ldx, Y, LI, tm, part, ldp, seg, mend);
-      return;
-    }
-  }
-  constexpr int K = multT_k<T>(D, false);
-  const int64_t cover = static_cast<int64_t>(kBlock) * K;
-  dim3 grid(static_cast<unsigned>((N + cover - 1) / cover), static_cast<unsigned>(S));
-  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D, false), T, false>), grid, dim3(kBlock), 0, s, X,
-                     ldx, Y, LI, tm, part, ldp, seg, mend);
+  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, Y,
+                     LI, TB, pm, part, ldp, jend, seg, mend);
 }
 
 template <typename T>
-using MultFn = void (*)(bool, const T*, int64_t, const T*, const T*, const uint32_t*, double*, int,
-                        int, int, int64_t, hipStream_t);
+using MultFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const uint32_t*, double*,
+                        int, int, int, int64_t, hipStream_t);
 template <typename T>
-using MultTFn = void (*)(bool, const T*, int64_t, const T*, const T*, const uint32_t*, double*,
-                         int64_t, int64_t, int64_t, int, int64_t, hipStream_t);
+using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const uint32_t*, double*,
+                         int64_t, int, int64_t, int64_t, int, hipStream_t);
 
 template <typename T, int... Ds>
 struct Table {
@@ -422,13 +576,90 @@ int mult_k_rt(int D) {
   return D > kMaxTemplD ? 1 : mult_k<T>(D);
 }
 template <typename T>
-int multT_k_rt(int D, bool skip) {
-  return D > kMaxTemplD ? 1 : multT_k<T>(D, skip && D >= 2);
+int multT_k_rt(int D) {
+  return D > kMaxTemplD ? 1 : multT_k<T>(D);
 }
 
 int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
 
 }  // namespace
+
+// Morton order of n points in [0,1]^d (row-major): b = min(16, 64/d) bits
+// per coordinate, interleaved from the top bit; ties keep the input order.
+std::vector<int32_t> morton_order(const double* pts, size_t n, size_t d) {
+  const int b = static_cast<int>(std::max<size_t>(1, std::min<size_t>(16, 64 / d)));
+  const size_t dd = std::min<size_t>(d, 64 / static_cast<size_t>(b));
+  const double scale = static_cast<double>(1u << b);
+  const uint32_t top = (1u << b) - 1u;
+  std::vector<std::pair<uint64_t, int32_t>> key(n);
+  std::vector<uint32_t> q(dd);
+  for (size_t i = 0; i < n; ++i) {
+    for (size_t k = 0; k < dd; ++k) {
+      const double v = pts[i * d + k] * scale;
+      q[k] = v <= 0.0 ? 0u : std::min<uint32_t>(top, static_cast<uint32_t>(v));
+    }
+    uint64_t c = 0;
+    for (int bit = b - 1; bit >= 0; --bit)
+      for (size_t k = 0; k < dd; ++k) c = (c << 1) | ((q[k] >> bit) & 1u);
+    key[i] = {c, static_cast<int32_t>(i)};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int32_t> order(n);
+  for (size_t i = 0; i < n; ++i) order[i] = key[i].second;
+  return order;
+}
+
+// Grid storage order: by the point's set of non-constant dimensions, then by
+// the Morton code of its support's centre (li: point-major (L, I) pairs).
+std::vector<int32_t> grid_order(const std::vector<double>& li, const std::vector<uint32_t>& pmask,
+                                size_t n, size_t d) {
+  std::vector<double> centre(n * d);
+  for (size_t j = 0; j < n; ++j)
+    for (size_t k = 0; k < d; ++k) {
+      const double L = li[(j * d + k) * 2], I = li[(j * d + k) * 2 + 1];
+      const double c = L > 0.0 ? I / L : 0.5;
+      centre[j * d + k] = std::min(1.0, std::max(0.0, c));
+    }
+  const std::vector<int32_t> mo = morton_order(centre.data(), n, d);
+  std::vector<int32_t> rank(n);
+  for (size_t i = 0; i < n; ++i) rank[mo[i]] = static_cast<int32_t>(i);
+  std::vector<int32_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    return pmask[a] != pmask[b] ? pmask[a] < pmask[b] : rank[a] < rank[b];
+  });
+  return order;
+}
+
+// Per 64-point tile and dimension the union of the points' supports
+// [(I - 1) / L, (I + 1) / L] (whole line for L = 0), from point-major (L, I)
+// pairs given as doubles, stored in T rounded outwards.
+template <typename T>
+std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d) {
+  const size_t ntiles = npad / kTG;
+  std::vector<T> tb(ntiles * d * 2);
+  for (size_t t = 0; t < ntiles; ++t)
+    for (size_t k = 0; k < d; ++k) {
+      double lo = 2.0, hi = -1.0;
+      for (size_t p = 0; p < static_cast<size_t>(kTG); ++p) {
+        const size_t e = ((t * kTG + p) * d + k) * 2;
+        const double L = li[e], I = li[e + 1];
+        if (L > 0.0) {
+          lo = std::min(lo, (I - 1.0) / L);
+          hi = std::max(hi, (I + 1.0) / L);
+        } else {
+          lo = -1.0;
+          hi = 2.0;
+        }
+      }
+      T tl = static_cast<T>(lo), th = static_cast<T>(hi);
+      if (static_cast<double>(tl) > lo) tl = std::nextafter(tl, static_cast<T>(-3));
+      if (static_cast<double>(th) < hi) th = std::nextafter(th, static_cast<T>(3));
+      tb[(t * d + k) * 2] = tl;
+      tb[(t * d + k) * 2 + 1] = th;
+    }
+  return tb;
+}
 
 // =================================================================== runtime
 
@@ -488,17 +719,25 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
   check_hip(hipEventCreate(&ev0_), "hipEventCreate");
   check_hip(hipEventCreate(&ev1_), "hipEventCreate");
 
-  Mpad_ = ((M_ + kDataPad - 1) / kDataPad) * kDataPad;
-  // pad by repeating the last row, transpose to [D][Mpad]
+  for (size_t i = 0; i < M_ * D_; ++i)
+    if (!(data[i] >= 0.0 && data[i] <= 1.0)) throw input_error("data points must lie in [0,1]^d");
+  // at least kDataSlack rows past the data: a wave's last group of points
+  // may start just below the end of the data
+  Mpad_ = ((M_ + kDataSlack + kDataPad - 1) / kDataPad) * kDataPad;
+  // storage order: Morton order of the points, so that consecutive points
+  // (one wave's) fill small boxes; dperm[stored position] = caller's row
+  std::vector<int32_t> dperm = morton_order(data, M_, D_);
+  // pad by repeating the last stored row, transpose to [D][Mpad]
   std::vector<double> soa(D_ * Mpad_);
   for (size_t d = 0; d < D_; ++d) {
     double* col = soa.data() + d * Mpad_;
-    for (size_t m = 0; m < M_; ++m) col[m] = data[m * D_ + d];
-    const double last = data[(M_ - 1) * D_ + d];
+    for (size_t m = 0; m < M_; ++m) col[m] = data[static_cast<size_t>(dperm[m]) * D_ + d];
+    const double last = col[M_ - 1];
     for (size_t m = M_; m < Mpad_; ++m) col[m] = last;
   }
-  for (double v : soa)
-    if (!(v >= 0.0 && v <= 1.0)) throw input_error("data points must lie in [0,1]^d");
+  check_hip(hipMalloc(&dperm_, M_ * sizeof(int32_t)), "hipMalloc(data order)");
+  check_hip(hipMemcpy(dperm_, dperm.data(), M_ * sizeof(int32_t), hipMemcpyHostToDevice),
+            "upload data order");
   if (serves(Precision::FP64)) {
     check_hip(hipMalloc(&X64_, soa.size() * sizeof(double)), "hipMalloc(X64)");
     check_hip(hipMemcpy(X64_, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice),
@@ -514,8 +753,8 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, scale_, perm_, tmask_, qmask_, A_, Y_, part_, hin_,
-                  hout_};
+  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, pmask_, dperm_, A_, Y_,
+                  part_, hin_, hout_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto* pool : {&tm_mult_, &tm_multT_})
@@ -543,77 +782,65 @@ void DeviceMultiEval::ensure(void** buf, size_t* cap, size_t bytes) {
 void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, size_t N) {
   if (N == 0) throw input_error("empty grid");
   if (N > (size_t(1) << 30)) throw input_error("grid too large");
-  // non-constant dimensions of each point; ModLinear level-1 factors are 1
+  // non-constant dimensions of each point (ModLinear level-1 factors are 1)
+  const bool mod = basis_ == Basis::ModLinear;
   std::vector<uint32_t> pmask(N);
-  const uint32_t full = D_ >= 32 ? 0xffffffffu : ((1u << D_) - 1u);
-  const bool skip = basis_ == Basis::ModLinear && D_ <= static_cast<size_t>(kMaxTemplD);
   for (size_t j = 0; j < N; ++j) {
     uint32_t m = 0;
-    for (size_t d = 0; d < D_; ++d)
-      if (!skip || level[j * D_ + d] > 1) m |= 1u << d;
-    pmask[j] = skip ? m : full;
+    for (size_t d = 0; d < D_ && d < 32; ++d)
+      if (!mod || level[j * D_ + d] > 1) m |= 1u << d;
+    pmask[j] = m;
   }
-  // storage order: sorted by the set of non-constant dimensions (stable)
-  std::vector<int32_t> order(N);
-  std::iota(order.begin(), order.end(), 0);
-  if (skip)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int32_t a, int32_t b) { return pmask[a] < pmask[b]; });
-  std::vector<uint32_t> lv(N * D_), ix(N * D_);
-  for (size_t s = 0; s < N; ++s)
-    for (size_t d = 0; d < D_; ++d) {
-      lv[s * D_ + d] = level[order[s] * D_ + d];
-      ix[s * D_ + d] = index[order[s] * D_ + d];
-    }
-  std::vector<double> li, sc;
-  encode_grid(basis_, D_, N, lv.data(), ix.data(), li, sc);
+  std::vector<double> li0, sc0;
+  encode_grid(basis_, D_, N, level, index, li0, sc0);
+  // storage order: by the set of non-constant dimensions, then by the Morton
+  // code of the support's centre, so that a 64-point tile holds points with
+  // the same dimensions in a small region
+  std::vector<int32_t> order = grid_order(li0, pmask, N, D_);
+  std::vector<double> li(N * D_ * 2), sc(N);
+  for (size_t s = 0; s < N; ++s) {
+    const size_t j = static_cast<size_t>(order[s]);
+    for (size_t e = 0; e < D_ * 2; ++e) li[s * D_ * 2 + e] = li0[j * D_ * 2 + e];
+    sc[s] = sc0[j];
+  }
   const size_t Npad = ((N + kGridPad - 1) / kGridPad) * kGridPad;
   // padded points: (L, I) = (0, 0) -> factor 1, weight 0 through A / ignored rows
   li.resize(Npad * D_ * 2, 0.0);
   sc.resize(Npad, 1.0);
-  // point-major [Npad][D][2] -> tile-major [Npad/64][D][64][2]
-  {
-    std::vector<double> t(li.size());
-    for (size_t j = 0; j < Npad; ++j)
-      for (size_t d = 0; d < D_; ++d) {
-        const size_t dst = (((j / kTG) * D_ + d) * kTG + (j % kTG)) * 2;
-        t[dst] = li[(j * D_ + d) * 2];
-        t[dst + 1] = li[(j * D_ + d) * 2 + 1];
-      }
-    li.swap(t);
-  }
-  std::vector<uint32_t> tm(Npad / kTG, 0u);
-  for (size_t s = 0; s < N; ++s) tm[s / kTG] |= pmask[order[s]];
+  std::vector<uint32_t> pm(Npad, 0u);
+  for (size_t s = 0; s < N; ++s) pm[s] = pmask[order[s]];
   check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   N_ = N;
   Npad_ = Npad;
-  skip_ = skip;
   ensure(reinterpret_cast<void**>(&scale_), &gridcap_scale_, Npad_ * sizeof(double));
   check_hip(hipMemcpy(scale_, sc.data(), Npad_ * sizeof(double), hipMemcpyHostToDevice),
             "upload scale");
-  ensure(&tmask_, &tmask_cap_, tm.size() * sizeof(uint32_t));
-  check_hip(hipMemcpy(tmask_, tm.data(), tm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
-            "upload tile masks");
-  std::vector<uint32_t> qm(Npad / 4, 0u);
-  for (size_t s = 0; s < N; ++s) qm[s / 4] |= pmask[order[s]];
-  ensure(&qmask_, &qmask_cap_, qm.size() * sizeof(uint32_t));
-  check_hip(hipMemcpy(qmask_, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
-            "upload sub-tile masks");
-  if (skip_) {
-    ensure(&perm_, &perm_cap_, N_ * sizeof(int32_t));
-    check_hip(hipMemcpy(perm_, order.data(), N_ * sizeof(int32_t), hipMemcpyHostToDevice),
-              "upload permutation");
-  }
+  ensure(&pmask_, &pmask_cap_, pm.size() * sizeof(uint32_t));
+  check_hip(hipMemcpy(pmask_, pm.data(), pm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+            "upload point masks");
+  ensure(&perm_, &perm_cap_, N_ * sizeof(int32_t));
+  check_hip(hipMemcpy(perm_, order.data(), N_ * sizeof(int32_t), hipMemcpyHostToDevice),
+            "upload permutation");
   if (serves(Precision::FP64)) {
     ensure(&LI64_, &gridcap_li64_, li.size() * sizeof(double));
     check_hip(hipMemcpy(LI64_, li.data(), li.size() * sizeof(double), hipMemcpyHostToDevice),
               "upload LI64");
+    const std::vector<double> tb = tile_boxes<double>(li, Npad_, D_);
+    ensure(&TB64_, &tbcap64_, tb.size() * sizeof(double));
+    check_hip(hipMemcpy(TB64_, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice),
+              "upload tile boxes");
   }
   if (serves(Precision::FP32)) {
     std::vector<float> l32(li.begin(), li.end());
     ensure(&LI32_, &gridcap_li32_, l32.size() * sizeof(float));
     check_hip(hipMemcpy(LI32_, l32.data(), l32.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload LI32");
+    // boxes of the FP32 (L, I) the FP32 kernels evaluate
+    const std::vector<double> l32d(l32.begin(), l32.end());
+    const std::vector<float> tb = tile_boxes<float>(l32d, Npad_, D_);
+    ensure(&TB32_, &tbcap32_, tb.size() * sizeof(float));
+    check_hip(hipMemcpy(TB32_, tb.data(), tb.size() * sizeof(float), hipMemcpyHostToDevice),
+              "upload tile boxes");
   }
 }
 
@@ -621,15 +848,14 @@ void DeviceMultiEval::record_start() { check_hip(hipEventRecord(ev0_, stream_), 
 void DeviceMultiEval::record_stop() { check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord"); }
 
 template <typename T>
-void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, double* d_result,
-                               hipEvent_t* tev) {
+void DeviceMultiEval::run_mult(const T* X, const T* LI, const T* TB, const double* d_alpha,
+                               double* d_result, hipEvent_t* tev) {
   hipStream_t s = stream_;
-  const int32_t* perm = skip_ ? static_cast<const int32_t*>(perm_) : nullptr;
   ensure(&A_, &A_cap_, Npad_ * sizeof(T));
   clear_launch_error();
   hipLaunchKernelGGL(prep_alpha_kernel<T>, dim3(ceil_div(Npad_, 256)), dim3(256), 0, s, d_alpha,
-                     scale_, perm, static_cast<T*>(A_), static_cast<int64_t>(N_),
-                     static_cast<int64_t>(Npad_));
+                     scale_, static_cast<const int32_t*>(perm_), static_cast<T*>(A_),
+                     static_cast<int64_t>(N_), static_cast<int64_t>(Npad_));
   check_hip(hipGetLastError(), "prep_alpha");
   const int K = mult_k_rt<T>(static_cast<int>(D_));
   const int ntiles = ceil_div(N_, kTG);
@@ -637,10 +863,13 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   int G = cfg_.mult_grid_chunks;
   if (G <= 0) {
     // many short blocks: the dispatcher then balances the uneven per-block
-    // work (the skip makes tiles cost different amounts) across all CUs
+    // work (culling makes blocks cost very different amounts) across all CUs
     const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : kBlocksPerCU * num_cus_;
     G = std::max(1, ceil_div(target, bx));
+    G = std::max(G, ceil_div(ntiles, kMaxChunkTiles));
     G = std::min(G, std::max(1, ntiles / 4));  // at least 4 tiles per chunk
+    // partials G x Mpad doubles within the budget
+    G = std::min<int64_t>(G, std::max<int64_t>(1, kPartialBudget / (Mpad_ * sizeof(double))));
   }
   G = std::max(1, std::min(G, ntiles));
   const int tiles_per = ceil_div(ntiles, G);
@@ -652,8 +881,8 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
     MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_));
-    f(skip_, X, static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_),
-      static_cast<const uint32_t*>(qmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
+    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A_),
+      static_cast<const uint32_t*>(pmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();
@@ -663,49 +892,58 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const double* d_alpha, d
   }
   check_hip(hipGetLastError(), "mult kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
+  // results back to the caller's row order
   clear_launch_error();
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M_, 256)), dim3(256), 0, s, P,
                      static_cast<int64_t>(Mpad_), G, static_cast<const double*>(nullptr),
-                     static_cast<const int32_t*>(nullptr), d_result, static_cast<int64_t>(M_));
+                     static_cast<const int32_t*>(dperm_), d_result, static_cast<int64_t>(M_));
   check_hip(hipGetLastError(), "reduce (mult)");
   last_G_ = G;
 }
 
 template <typename T>
-void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, double* d_result,
-                                hipEvent_t* tev) {
+void DeviceMultiEval::run_multT(const T* X, const T* LI, const T* TB, const double* d_y,
+                                double* d_result, hipEvent_t* tev) {
   hipStream_t s = stream_;
   ensure(&Y_, &Y_cap_, Mpad_ * sizeof(T));
   clear_launch_error();
   hipLaunchKernelGGL(prep_y_kernel<T>, dim3(ceil_div(Mpad_, 256)), dim3(256), 0, s, d_y,
-                     static_cast<T*>(Y_), static_cast<int64_t>(M_), static_cast<int64_t>(Mpad_));
+                     static_cast<const int32_t*>(dperm_), static_cast<T*>(Y_),
+                     static_cast<int64_t>(M_), static_cast<int64_t>(Mpad_));
   check_hip(hipGetLastError(), "prep_y");
-  const int K = multT_k_rt<T>(static_cast<int>(D_), skip_);
-  const int64_t cover = static_cast<int64_t>(kBlock) * K;
-  const int bx = ceil_div(N_, cover);
-  // data end: only tiles that contain real points (the rest of Mpad is padding)
-  const int64_t mtiles = ceil_div(M_, kTM);
-  const int64_t mend = mtiles * kTM;
+  const bool templ = D_ <= static_cast<size_t>(kMaxTemplD);
+  const int ntiles = ceil_div(N_, kTG);
+  const int jend = ntiles * kTG;
+  // grid blocks: chunks of kChunkT points (templated kernel) or kBlock points
+  const int bx = templ ? ceil_div(jend, kChunkT) : ceil_div(N_, kBlock);
+  // Data are split into groups (templated kernel: one wave's 64*K points;
+  // generic kernel: kTM points); a segment is 4 waves x gpw groups, so the
+  // four waves of a block get equal shares. mend = whole groups past M (the
+  // padding holds at least kDataSlack rows past M).
+  const int64_t group = templ ? 64 * multT_k_rt<T>(static_cast<int>(D_)) : kTM;
+  const int64_t ngroups = (static_cast<int64_t>(M_) + group - 1) / group;
+  const int64_t mend = ngroups * group;
+  const int64_t waves = templ ? kBlock / 64 : 1;
   int S = cfg_.multT_data_segs;
   if (S <= 0) {
-    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : kBlocksPerCU * num_cus_;
+    const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks
+                                              : (templ ? kBlocksPerCUT : kBlocksPerCU) * num_cus_;
     S = std::max(1, ceil_div(target, bx));
-    S = static_cast<int>(std::min<int64_t>(S, std::max<int64_t>(1, mtiles / 4)));
+    S = std::min<int64_t>(S, std::max<int64_t>(1, kPartialBudget / (Npad_ * sizeof(double))));
   }
-  S = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>(S, mtiles)));
-  const int64_t tiles_per = (mtiles + S - 1) / S;
-  S = static_cast<int>((mtiles + tiles_per - 1) / tiles_per);
-  const int64_t seg = tiles_per * kTM;
+  const int64_t gpw = std::max<int64_t>(1, (ngroups + waves * S - 1) / (waves * S));
+  const int64_t seg = waves * gpw * group;
+  S = ceil_div(mend, seg);
   ensure(&part_, &part_cap_, static_cast<size_t>(S) * Npad_ * sizeof(double));
   double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
-  if (D_ <= static_cast<size_t>(kMaxTemplD)) {
+  if (templ) {
     MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_));
-    f(skip_, X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI,
-      static_cast<const uint32_t*>(tmask_), P, static_cast<int64_t>(Npad_), seg, mend, S,
-      static_cast<int64_t>(N_), s);
+    f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
+      static_cast<const uint32_t*>(pmask_), P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
+      s);
   } else {
-    dim3 grid(static_cast<unsigned>(ceil_div(N_, kBlock)), static_cast<unsigned>(S));
+    dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(S));
     clear_launch_error();
     hipLaunchKernelGGL(multT_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
                        static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, P,
@@ -713,11 +951,11 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const double* d_y, doub
   }
   check_hip(hipGetLastError(), "multTranspose kernel");
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
+  // results back to the caller's grid order, times the per-point scale
   clear_launch_error();
   hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N_, 256)), dim3(256), 0, s, P,
-                     static_cast<int64_t>(Npad_), S, scale_,
-                     skip_ ? static_cast<const int32_t*>(perm_) : nullptr, d_result,
-                     static_cast<int64_t>(N_));
+                     static_cast<int64_t>(Npad_), S, scale_, static_cast<const int32_t*>(perm_),
+                     d_result, static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "reduce (multTranspose)");
   last_S_ = S;
 }
@@ -730,11 +968,11 @@ void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Preci
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
   if (p == Precision::FP64)
-    run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_), d_alpha,
-                     d_result, tp ? tev : nullptr);
+    run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
+                     static_cast<const double*>(TB64_), d_alpha, d_result, tp ? tev : nullptr);
   else
-    run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_), d_alpha,
-                    d_result, tp ? tev : nullptr);
+    run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
+                    static_cast<const float*>(TB32_), d_alpha, d_result, tp ? tev : nullptr);
   record_stop();
 }
 
@@ -746,11 +984,11 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
   if (p == Precision::FP64)
-    run_multT<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_), d_y,
-                      d_result, tp ? tev : nullptr);
+    run_multT<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
+                      static_cast<const double*>(TB64_), d_y, d_result, tp ? tev : nullptr);
   else
-    run_multT<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_), d_y,
-                     d_result, tp ? tev : nullptr);
+    run_multT<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
+                     static_cast<const float*>(TB32_), d_y, d_result, tp ? tev : nullptr);
   record_stop();
 }
 
