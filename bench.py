@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--serve", action="store_true",
                     help="run the served-loop workload (64 models, Poisson requests) instead")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="serve with the untuned default tile shapes")
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served workload (requests/s)")
     ap.add_argument("--solve", action="store_true",
@@ -215,10 +217,19 @@ def serve_bench(args):
                      ("one_launch_per_request_fp64",
                       dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
                            use_graph=False))):
+        kw["autotune"] = not args.no_autotune
+        t = time.perf_counter()
         srv = capi.Server(**kw)
         ids = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
                for m in c["models"]]
+        admit_s = time.perf_counter() - t
+        print("[serve] %s: %d models admitted in %.1f s" % (name, len(ids), admit_s),
+              file=sys.stderr, flush=True)
         res, outs = drive_server(srv, ids, c)
+        res["admit_and_tune_s"] = admit_s
+        res["tuned_shapes"] = {"%s_d%d" % ("modlinear" if b else "linear", d): srv.tuned_shape(b, d)
+                               for b, d in sorted({(m["basis"], m["dim"]) for m in c["models"]})
+                               if srv.tuned_shape(b, d) is not None}
         res["achieved_tflops_6d"] = flops * res["requests_done"] / len(c["q"]) / res["span_s"] / 1e12
         runs[name] = res
         outputs[name] = outs

@@ -208,8 +208,17 @@ typedef struct sgpp_hip_server_config {
   int max_requests_per_batch;   /* 1 = one launch per request; 0 = 65536 */
   int blocks;                   /* persistent grid size; 0 = 8 per CU */
   int use_graph;                /* replay the batch kernels from a hipGraph */
-  double item_work;             /* lane-operations per work item; 0 = 2^25 */
+  double item_work;             /* untuned: lane-operations per work item; 0 = 2^25 */
+  int autotune;                 /* time tile shapes per (basis, dimension) at admission */
+  const char* autotune_cache;   /* file of tuned shapes (read, then updated); NULL = none */
 } sgpp_hip_server_config;
+
+/* A tuned shape: grid points per work item and the measured seconds per
+ * (query x grid point). */
+typedef struct sgpp_hip_server_shape {
+  int chunk;
+  double cost;
+} sgpp_hip_server_shape;
 
 enum {
   SGPP_HIP_REQ_QUEUED = 0,
@@ -251,6 +260,12 @@ sgpp_hip_status sgpp_hip_server_cancel(sgpp_hip_server* s, uint64_t request_id);
 sgpp_hip_status sgpp_hip_server_inject_error(sgpp_hip_server* s);
 sgpp_hip_status sgpp_hip_server_drain(sgpp_hip_server* s);
 sgpp_hip_status sgpp_hip_server_stats(sgpp_hip_server* s, sgpp_hip_server_statistics* out);
+/* the tuned shape of (basis, dim); *found = 0 when that pair is not tuned.
+ * No reference counterpart (the reference has no serving loop); its OpenCL
+ * operator tuner is StaticParameterTuner (datadriven/src/sgpp/datadriven/
+ * application/StaticParameterTuner.cpp:234-367). */
+sgpp_hip_status sgpp_hip_server_tuned_shape(sgpp_hip_server* s, int basis, int dim, int* found,
+                                            sgpp_hip_server_shape* out);
 
 /* --------------------------------------------------------- learner
  * The regression learner that drives the path: regular grid -> (refine ->

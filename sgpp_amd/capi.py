@@ -172,6 +172,8 @@ SIGNATURES = {
     "sgpp_hip_server_inject_error": (_st, [_p]),
     "sgpp_hip_server_drain": (_st, [_p]),
     "sgpp_hip_server_stats": (_st, [_p, ctypes.c_void_p]),
+    "sgpp_hip_server_tuned_shape": (_st, [_p, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
     "sgpp_hip_learner_create": (_st, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_p)]),
     "sgpp_hip_learner_destroy": (_st, [_p]),
     "sgpp_hip_learner_train": (_st, [_p, _dp, _sz, _sz, _dp, ctypes.POINTER(GridConfig),
@@ -434,7 +436,12 @@ class ServerConfig(ctypes.Structure):
                 ("max_items", ctypes.c_size_t), ("max_partial_doubles", ctypes.c_size_t),
                 ("max_inflight", ctypes.c_int), ("max_requests_per_batch", ctypes.c_int),
                 ("blocks", ctypes.c_int), ("use_graph", ctypes.c_int),
-                ("item_work", ctypes.c_double)]
+                ("item_work", ctypes.c_double), ("autotune", ctypes.c_int),
+                ("autotune_cache", ctypes.c_char_p)]
+
+
+class ServerShape(ctypes.Structure):
+    _fields_ = [("chunk", ctypes.c_int), ("cost", ctypes.c_double)]
 
 
 class ServerStatistics(ctypes.Structure):
@@ -448,10 +455,11 @@ class Server:
 
     def __init__(self, precision=PREC_FP32, memory_budget=0, max_batch_queries=0, max_items=0,
                  max_partial_doubles=0, max_inflight=2, max_requests_per_batch=0, blocks=0,
-                 use_graph=True, item_work=0.0, device=0):
+                 use_graph=True, item_work=0.0, device=0, autotune=False, autotune_cache=None):
+        self._cache = autotune_cache.encode() if autotune_cache else None
         cfg = ServerConfig(device, precision, memory_budget, max_batch_queries, max_items,
                            max_partial_doubles, max_inflight, max_requests_per_batch, blocks,
-                           int(use_graph), item_work)
+                           int(use_graph), item_work, int(autotune), self._cache)
         h = _p()
         check(F["sgpp_hip_server_create"](ctypes.cast(ctypes.byref(cfg), ctypes.c_void_p),
                                           ctypes.byref(h)))
@@ -518,6 +526,16 @@ class Server:
         s = ServerStatistics()
         check(F["sgpp_hip_server_stats"](self.h, ctypes.cast(ctypes.byref(s), ctypes.c_void_p)))
         return {k: getattr(s, k) for k, _ in ServerStatistics._fields_}
+
+    def tuned_shape(self, basis, dim):
+        """{"chunk", "cost"} of the tuned shape of (basis, dim), or None."""
+        found = ctypes.c_int(0)
+        sh = ServerShape()
+        check(F["sgpp_hip_server_tuned_shape"](self.h, basis, dim, ctypes.byref(found),
+                                               ctypes.cast(ctypes.byref(sh), ctypes.c_void_p)))
+        if not found.value:
+            return None
+        return {"chunk": sh.chunk, "cost": sh.cost}
 
 
 # ------------------------------------------------------------- learner

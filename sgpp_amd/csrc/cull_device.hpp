@@ -1,0 +1,265 @@
+// Device building blocks shared by the multi-evaluation kernels and the
+// evaluation server: the hat-function step, DPP wave reductions, and the
+// culling tests of grid tiles and grid points against a wave's box of
+// data points.
+//
+// Culling is exact: a grid point is dropped only when, for some dimension,
+// fma(L, hi, -I) < -1 or fma(L, lo, -I) > 1 over the wave's box [lo, hi].
+// fma rounds monotonically in x (L >= 0), so the kernel's own u = fma(L, x, -I)
+// then has |u| >= 1 for every x in the box, and its factor
+// clamp01(p - p*|u|) is exactly 0.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace sgpp_hip {
+
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
+__device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
+
+// running product: p * max(0, 1 - |u|) for p in [0,1]
+template <typename T>
+__device__ __forceinline__ T hat_step(T p, T L, T x, T I) {
+  const T u = fmaT(L, x, -I);
+  return clamp01(fmaT(-p, absT(u), p));
+}
+
+// ------------------------------------------------------- wave helpers
+// Cross-lane moves with DPP row controls (VALU operand modifiers, no LDS
+// traffic). Lanes whose source is outside the row, or whose row is not in
+// RowMask, receive `old`.
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ float dpp(float old, float v) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), Ctrl, RowMask, 0xf, false));
+}
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ double dpp(double old, double v) {
+  const long long o = __double_as_longlong(old), s = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), Ctrl, RowMask,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32),
+                                             Ctrl, RowMask, 0xf, false);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// Reduction over the 64 lanes; the result is complete in lane 63 only.
+// Steps: swap within pairs and halves of each quad (every lane then holds its
+// quad's value), shift right by 4 and 8 within each row of 16 (lane 15 of a
+// row holds the row's value), broadcast lane 15 into the next row (lanes 31
+// and 63 hold two rows each), broadcast lane 31 into rows 2-3 (lane 63).
+// For sums the lanes outside a step's reach add 0; for min/max they combine
+// with themselves.
+template <typename T>
+__device__ __forceinline__ T sum_to_lane63(T v) {
+  v += dpp<0xb1>(T(0), v);        // quad_perm [1,0,3,2]
+  v += dpp<0x4e>(T(0), v);        // quad_perm [2,3,0,1]
+  v += dpp<0x114>(T(0), v);       // row_shr:4
+  v += dpp<0x118>(T(0), v);       // row_shr:8
+  v += dpp<0x142, 0xa>(T(0), v);  // row_bcast:15 into rows 1, 3
+  v += dpp<0x143, 0xc>(T(0), v);  // row_bcast:31 into rows 2, 3
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T min_to_lane63(T v) {
+  v = fmin(v, dpp<0xb1>(v, v));
+  v = fmin(v, dpp<0x4e>(v, v));
+  v = fmin(v, dpp<0x114>(v, v));
+  v = fmin(v, dpp<0x118>(v, v));
+  v = fmin(v, dpp<0x142, 0xa>(v, v));
+  v = fmin(v, dpp<0x143, 0xc>(v, v));
+  return v;
+}
+template <typename T>
+__device__ __forceinline__ T max_to_lane63(T v) {
+  v = fmax(v, dpp<0xb1>(v, v));
+  v = fmax(v, dpp<0x4e>(v, v));
+  v = fmax(v, dpp<0x114>(v, v));
+  v = fmax(v, dpp<0x118>(v, v));
+  v = fmax(v, dpp<0x142, 0xa>(v, v));
+  v = fmax(v, dpp<0x143, 0xc>(v, v));
+  return v;
+}
+// lane 63's value, moved to a scalar register
+__device__ __forceinline__ float lane63(float v) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ __forceinline__ double lane63(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), 63);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
+// The box [lo, hi] of a wave's 64*K data points in the first DC dimensions
+// (wave-uniform: scalar registers). Culling tests only these dimensions;
+// testing fewer dimensions keeps more grid points, never drops one that can
+// be nonzero.
+template <int DC, typename T>
+struct WaveBox {
+  T l[DC], h[DC];
+};
+
+template <int DC, int D, int K, typename T>
+__device__ __forceinline__ void wave_box(const T (&x)[K][D], WaveBox<DC, T>& box) {
+  static_assert(DC <= D, "tested dimensions");
+#pragma unroll
+  for (int d = 0; d < DC; ++d) {
+    T a = x[0][d], b = x[0][d];
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      a = fmin(a, x[k][d]);
+      b = fmax(b, x[k][d]);
+    }
+    box.l[d] = lane63(min_to_lane63(a));
+    box.h[d] = lane63(max_to_lane63(b));
+  }
+}
+
+// False only if the factor max(0, 1 - |L x - I|) (L >= 0) is exactly zero for
+// every x in [lo, hi]: in exact arithmetic that holds when L*hi - I <= -1 or
+// L*lo - I >= 1. fma rounds monotonically, so "fma(L, hi, -I) < -1" implies
+// L*hi - I < -1: a point is only ever dropped when the kernel's own
+// u = fma(L, x, -I) has |u| >= 1 for all x in the box.
+template <typename T>
+__device__ __forceinline__ bool may_hit(T L, T I, T lo, T hi) {
+  return !(fmaT(L, hi, -I) < T(-1)) && !(fmaT(L, lo, -I) > T(1));
+}
+
+// dimensions per batch of bound loads in the tile and point tests
+constexpr int kTestBatch = 8;
+
+// Tile pre-test: the tile's union of supports TB[t][d] = [lo, hi] (FP32,
+// rounded outwards; compared in T, which is exact) against the wave's box.
+// The bounds are loaded in batches of kTestBatch dimensions and all tests of
+// a batch run after its loads (no short-circuit), so the loads are issued
+// together.
+// TB rows hold D dimensions; the first DC are tested.
+template <int DC, int D, typename T>
+__device__ __forceinline__ bool tile_may_hit(const float* __restrict__ TB, int64_t t,
+                                             const WaveBox<DC, T>& box) {
+  const float* tb = TB + t * D * 2;
+  unsigned h = 1u;
+#pragma unroll
+  for (int d0 = 0; d0 < DC; d0 += kTestBatch) {
+    float b[2 * kTestBatch];
+#pragma unroll
+    for (int e = 0; e < 2 * kTestBatch; ++e)
+      if (2 * d0 + e < 2 * DC) b[e] = tb[2 * d0 + e];
+#pragma unroll
+    for (int dd = 0; dd < kTestBatch; ++dd)
+      if (d0 + dd < DC)
+        h &= static_cast<unsigned>(static_cast<T>(b[2 * dd]) < box.h[d0 + dd]) &
+             static_cast<unsigned>(static_cast<T>(b[2 * dd + 1]) > box.l[d0 + dd]);
+  }
+  return h != 0u;
+}
+
+// One lane per point of the tile: which points may be nonzero in the box
+// (the lane's (L, I) row is read in batches of kTestBatch dimensions).
+// Rows hold D dimensions; the first DC are tested.
+template <int DC, int D, typename T>
+__device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ tile, int lane,
+                                                        const WaveBox<DC, T>& box) {
+  const T* row = tile + lane * D * 2;
+  unsigned h = 1u;
+#pragma unroll
+  for (int d0 = 0; d0 < DC; d0 += kTestBatch) {
+    T r[2 * kTestBatch];
+#pragma unroll
+    for (int e = 0; e < 2 * kTestBatch; ++e)
+      if (2 * d0 + e < 2 * DC) r[e] = row[2 * d0 + e];
+#pragma unroll
+    for (int dd = 0; dd < kTestBatch; ++dd)
+      if (d0 + dd < DC)
+        h &= static_cast<unsigned>(may_hit(r[2 * dd], r[2 * dd + 1], box.l[d0 + dd], box.h[d0 + dd]));
+  }
+  return __ballot(h != 0u);
+}
+
+// Takes up to J points from the hit mask (wave-uniform). Missing slots
+// repeat slot 0 and are flagged unused; w = the points' weights W[j0 + jj]
+// (0 for unused slots; none without kW), loaded before the products so that
+// their latency overlaps the evaluation.
+template <int J, bool kW, typename T>
+__device__ __forceinline__ void take_hits(unsigned long long& bal, const T* __restrict__ W, int j0,
+                                          int (&jj)[J], bool (&used)[J], T (&w)[J]) {
+#pragma unroll
+  for (int q = 0; q < J; ++q) {
+    if (bal) {
+      const int b = __builtin_ctzll(bal);
+      bal &= bal - 1;
+      jj[q] = b;
+      used[q] = true;
+      w[q] = kW ? W[j0 + b] : T(0);
+    } else {
+      jj[q] = jj[0];
+      used[q] = false;
+      w[q] = T(0);
+    }
+  }
+}
+
+// bit pattern of a value (L, I >= 0 are zero exactly when their bits are zero)
+__device__ __forceinline__ uint32_t bits_of(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ uint64_t bits_of(double v) {
+  return static_cast<uint64_t>(__double_as_longlong(v));
+}
+
+// An (L, I) pair, loaded as one 8- or 16-byte vector.
+template <typename T>
+struct alignas(2 * sizeof(T)) LIPair {
+  T L, I;
+};
+
+// Products p[q][k] = phi_{jj[q]}(x_k) of the J taken points of a tile. The
+// points' (L, I) pairs are read in batches of DB dimensions (wave-uniform
+// addresses: wide scalar loads). A dimension in which every taken point has
+// (L, I) = (0, 0) (a ModLinear level-1 factor, exactly 1) is skipped. The
+// batch's skip mask is computed from all its loaded pairs before any test, so
+// all loads of the batch are issued together, ahead of the branches.
+template <int D, int J, int K, typename T>
+__device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int (&jj)[J],
+                                           const T (&x)[K][D], T (&p)[J][K]) {
+  // dimensions per batch: the batch's pairs take at most 48 scalar registers
+  constexpr int kWordsPerDim = J * 2 * static_cast<int>(sizeof(T)) / 4;
+  constexpr int DB = D < 48 / kWordsPerDim ? D : 48 / kWordsPerDim;
+  const LIPair<T>* rows = reinterpret_cast<const LIPair<T>*>(tile);
+#pragma unroll
+  for (int q = 0; q < J; ++q)
+#pragma unroll
+    for (int k = 0; k < K; ++k) p[q][k] = T(1);
+#pragma unroll
+  for (int d0 = 0; d0 < D; d0 += DB) {
+    LIPair<T> li[J][DB];
+#pragma unroll
+    for (int q = 0; q < J; ++q)
+#pragma unroll
+      for (int dd = 0; dd < DB; ++dd)
+        if (d0 + dd < D) li[q][dd] = rows[jj[q] * D + d0 + dd];
+    uint32_t live = 0u;
+#pragma unroll
+    for (int dd = 0; dd < DB; ++dd) {
+      if (d0 + dd >= D) continue;
+      auto any = bits_of(li[0][dd].L) | bits_of(li[0][dd].I);
+#pragma unroll
+      for (int q = 1; q < J; ++q) any |= bits_of(li[q][dd].L) | bits_of(li[q][dd].I);
+      live |= (any != 0 ? 1u : 0u) << dd;
+    }
+#pragma unroll
+    for (int dd = 0; dd < DB; ++dd) {
+      const int d = d0 + dd;
+      if (d >= D || !((live >> dd) & 1u)) continue;
+#pragma unroll
+      for (int q = 0; q < J; ++q)
+#pragma unroll
+        for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[q][dd].L, x[k][d], li[q][dd].I);
+    }
+  }
+}
+
+}  // namespace sgpp_hip

@@ -62,6 +62,19 @@ void encode_grid(Basis basis, size_t dim, size_t n, const uint32_t* level, const
 // Morton order of n points in [0,1]^d (row-major); ties keep input order.
 std::vector<int32_t> morton_order(const double* pts, size_t n, size_t d);
 
+// Grid storage order for culling: by the point's set of non-constant
+// dimensions (pmask), then by the Morton code of its support's centre.
+// li: point-major (L, I) pairs from encode_grid.
+std::vector<int32_t> grid_order(const std::vector<double>& li, const std::vector<uint32_t>& pmask,
+                                size_t n, size_t d);
+
+// Per 64-point tile and dimension, the union of the points' supports
+// [(I - 1) / L, (I + 1) / L] (the whole line for L = 0), rounded outwards to
+// T. li: point-major (L, I) pairs of npad points (npad a multiple of 64).
+// Returns [npad/64][d][2].
+template <typename T>
+std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d);
+
 class DeviceMultiEval {
  public:
   // data_rowmajor: M x d, FP64 row-major (the caller's DataMatrix layout).
@@ -114,20 +127,17 @@ class DeviceMultiEval {
   void record_start();
   void record_stop();
   template <typename T>
-  void run_mult(const T* X, const T* LI, const T* TB, const double* d_alpha, double* d_result,
+  void run_mult(const T* X, const T* LI, const float* TB, const double* d_alpha, double* d_result,
                 hipEvent_t* tev);
   template <typename T>
-  void run_multT(const T* X, const T* LI, const T* TB, const double* d_y, double* d_result,
+  void run_multT(const T* X, const T* LI, const float* TB, const double* d_y, double* d_result,
                  hipEvent_t* tev);
 
   // Storage orders. Data: Morton order, dperm_[stored row] = caller's row.
   // Grid: by each point's set of non-constant dimensions, then Morton order
-  // of the support centres; perm_[stored position] = caller's index,
-  // pmask_[stored position] = that set (all dimensions for Linear, 0 for
-  // padding).
+  // of the support centres; perm_[stored position] = caller's index.
   void* dperm_ = nullptr;
   void* perm_ = nullptr;   size_t perm_cap_ = 0;
-  void* pmask_ = nullptr;  size_t pmask_cap_ = 0;
 
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;
@@ -142,7 +152,8 @@ class DeviceMultiEval {
   void* X64_ = nullptr;
   void* X32_ = nullptr;
   // grid pairs, point-major [Npad][D][2], per-point scale [Npad] and
-  // per-tile (64 points) boxes of the supports [Npad/64][D][2]
+  // per-tile (64 points) boxes of the supports [Npad/64][D][2], FP32 rounded
+  // outwards for both precisions
   void* LI64_ = nullptr;
   void* LI32_ = nullptr;
   void* TB64_ = nullptr;

@@ -8,22 +8,26 @@
 // which the compiler emits as v_fma + v_fma(neg, abs, clamp). This is VALU
 // work (no contraction), so there is no MFMA anywhere in this file.
 //
-// mult (r = B alpha): lanes own data points (K per lane, coordinates held in
-//   registers for the whole launch); grid tiles of 64 points are staged in
-//   LDS and read as wave-uniform broadcasts. Work is split over data tiles
-//   (blockIdx.x) and G grid chunks (blockIdx.y); the G FP64 partial rows are
-//   summed in a fixed order by reduce_parts.
-// multTranspose (a = B^T y): lanes own grid points (K per lane, L/I in
-//   registers); data tiles of 128 points (coordinates + y) are staged in LDS
-//   and broadcast. Work is split over grid tiles (blockIdx.x) and S data
-//   segments (blockIdx.y); the S FP64 partial rows are summed in a fixed
-//   order. No atomics are used anywhere: results are deterministic.
-// Constant factors: a ModLinear level-1 factor is exactly 1. For ModLinear
-//   grids the points are kept sorted by their set of non-constant dimensions
-//   (an internal permutation, undone on input and output), every 64-point
-//   tile carries the union of its points' sets, and mult skips the other
-//   dimensions with a wave-uniform scalar branch. Skipping a factor of
-//   exactly 1 leaves every result bit-identical.
+// Layout: data points SoA [D][Mpad] in Morton order (dperm maps a stored row
+// to the caller's row); grid points point-major (L, I) rows [Npad][D][2],
+// sorted by their set of non-constant dimensions and the Morton code of their
+// support centre (perm maps a stored point to the caller's index), with
+// per-64-point-tile support boxes and per-point dimension masks.
+//
+// Both kernels give lanes K data points each (coordinates in registers). A
+// wave takes the box of its 64*K points, skips grid tiles whose box misses
+// it, tests the remaining tiles' points against it with one ballot and
+// evaluates the points that may be nonzero, J at a time, over the union of
+// their non-constant dimensions (ModLinear level-1 factors are exactly 1).
+// Every dropped term is exactly zero (cull_device.hpp), so results equal a
+// full evaluation.
+//   mult (r = B alpha): blocks = data tiles (x) x G grid chunks (y); the G
+//     FP64 partial rows are summed in a fixed order by reduce_parts.
+//   multTranspose (a = B^T y): blocks = grid chunks of kChunkT points (x) x
+//     S data segments (y); per taken grid point the wave's sum over its lanes
+//     (DPP) is added to the wave's row of FP64 LDS accumulators; the four rows
+//     and the S partial rows are summed in a fixed order. No global atomics:
+//     results are deterministic.
 // Precision: FP64 throughout, or FP32 basis products and per-tile sums with
 //   the per-tile partials accumulated in FP64 registers and FP64 partials.
 //
@@ -41,6 +45,7 @@
 #include <numeric>
 #include <sstream>
 
+#include "cull_device.hpp"
 #include "meval_device.hpp"
 
 namespace sgpp_hip {
@@ -64,20 +69,6 @@ constexpr int64_t kPartialBudget = int64_t(512) << 20;
 // data rows past M kept in the padded data (a wave's last group of 64*K
 // points may start just below M)
 constexpr size_t kDataSlack = 1024;
-
-__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
-__device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
-__device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
-__device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
-
-// running product: p * max(0, 1 - |u|) for p in [0,1]
-template <typename T>
-__device__ __forceinline__ T hat_step(T p, T L, T x, T I) {
-  const T u = fmaT(L, x, -I);
-  return clamp01(fmaT(-p, absT(u), p));
-}
 
 // Points per lane (K) and grid points evaluated together (J). Values from
 // variant sweeps on the d=10 ModLinear and d=8 Linear workloads.
@@ -105,167 +96,6 @@ constexpr int kChunkT = 1024;
 // whole row with one wide scalar load; tile t = points [64t, 64t+64)).
 __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * D + d) * 2; }
 
-// ------------------------------------------------------- wave helpers
-// Cross-lane moves with DPP row controls (VALU operand modifiers, no LDS
-// traffic). Lanes whose source is outside the row, or whose row is not in
-// RowMask, receive `old`.
-template <int Ctrl, int RowMask = 0xf>
-__device__ __forceinline__ float dpp(float old, float v) {
-  return __int_as_float(
-      __builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), Ctrl, RowMask, 0xf, false));
-}
-template <int Ctrl, int RowMask = 0xf>
-__device__ __forceinline__ double dpp(double old, double v) {
-  const long long o = __double_as_longlong(old), s = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(static_cast<int>(o), static_cast<int>(s), Ctrl, RowMask,
-                                             0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(static_cast<int>(o >> 32), static_cast<int>(s >> 32),
-                                             Ctrl, RowMask, 0xf, false);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-// Reduction over the 64 lanes; the result is complete in lane 63 only.
-// Steps: swap within pairs and halves of each quad (every lane then holds its
-// quad's value), shift right by 4 and 8 within each row of 16 (lane 15 of a
-// row holds the row's value), broadcast lane 15 into the next row (lanes 31
-// and 63 hold two rows each), broadcast lane 31 into rows 2-3 (lane 63).
-// For sums the lanes outside a step's reach add 0; for min/max they combine
-// with themselves.
-template <typename T>
-__device__ __forceinline__ T sum_to_lane63(T v) {
-  v += dpp<0xb1>(T(0), v);        // quad_perm [1,0,3,2]
-  v += dpp<0x4e>(T(0), v);        // quad_perm [2,3,0,1]
-  v += dpp<0x114>(T(0), v);       // row_shr:4
-  v += dpp<0x118>(T(0), v);       // row_shr:8
-  v += dpp<0x142, 0xa>(T(0), v);  // row_bcast:15 into rows 1, 3
-  v += dpp<0x143, 0xc>(T(0), v);  // row_bcast:31 into rows 2, 3
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T min_to_lane63(T v) {
-  v = fmin(v, dpp<0xb1>(v, v));
-  v = fmin(v, dpp<0x4e>(v, v));
-  v = fmin(v, dpp<0x114>(v, v));
-  v = fmin(v, dpp<0x118>(v, v));
-  v = fmin(v, dpp<0x142, 0xa>(v, v));
-  v = fmin(v, dpp<0x143, 0xc>(v, v));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T max_to_lane63(T v) {
-  v = fmax(v, dpp<0xb1>(v, v));
-  v = fmax(v, dpp<0x4e>(v, v));
-  v = fmax(v, dpp<0x114>(v, v));
-  v = fmax(v, dpp<0x118>(v, v));
-  v = fmax(v, dpp<0x142, 0xa>(v, v));
-  v = fmax(v, dpp<0x143, 0xc>(v, v));
-  return v;
-}
-// lane 63's value, moved to a scalar register
-__device__ __forceinline__ float lane63(float v) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
-}
-__device__ __forceinline__ double lane63(double v) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b & 0xffffffffLL), 63);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
-}
-// Box [lo, hi] per dimension of the wave's 64*K data points.
-template <int D, int K, typename T>
-__device__ __forceinline__ void wave_box(const T (&x)[K][D], T (&lo)[D], T (&hi)[D]) {
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    T a = x[0][d], b = x[0][d];
-#pragma unroll
-    for (int k = 1; k < K; ++k) {
-      a = fmin(a, x[k][d]);
-      b = fmax(b, x[k][d]);
-    }
-    lo[d] = lane63(min_to_lane63(a));
-    hi[d] = lane63(max_to_lane63(b));
-  }
-}
-
-// False only if the factor max(0, 1 - |L x - I|) (L >= 0) is exactly zero for
-// every x in [lo, hi]: in exact arithmetic that holds when L*hi - I <= -1 or
-// L*lo - I >= 1. fma rounds monotonically, so "fma(L, hi, -I) < -1" implies
-// L*hi - I < -1: a point is only ever dropped when the kernel's own
-// u = fma(L, x, -I) has |u| >= 1 for all x in the box.
-template <typename T>
-__device__ __forceinline__ bool may_hit(T L, T I, T lo, T hi) {
-  return !(fmaT(L, hi, -I) < T(-1)) && !(fmaT(L, lo, -I) > T(1));
-}
-
-// Tile pre-test: the tile's union of supports TB[t][d] = [lo, hi] (rounded
-// outwards) against the wave's data box. All 2*D bounds are loaded before any
-// is tested (no short-circuit), so the loads are issued together.
-template <int D, typename T>
-__device__ __forceinline__ bool tile_may_hit(const T* __restrict__ TB, int64_t t, const T (&lo)[D],
-                                             const T (&hi)[D]) {
-  const T* tb = TB + t * D * 2;
-  T b[2 * D];
-#pragma unroll
-  for (int e = 0; e < 2 * D; ++e) b[e] = tb[e];
-  unsigned h = 1u;
-#pragma unroll
-  for (int d = 0; d < D; ++d)
-    h &= static_cast<unsigned>(b[2 * d] < hi[d]) & static_cast<unsigned>(b[2 * d + 1] > lo[d]);
-  return h != 0u;
-}
-
-// One lane per point of the tile: which points may be nonzero in the box
-// (the lane's whole (L, I) row is loaded before the tests).
-template <int D, typename T>
-__device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ tile, int lane,
-                                                        const T (&lo)[D], const T (&hi)[D]) {
-  const T* row = tile + lane * D * 2;
-  T r[2 * D];
-#pragma unroll
-  for (int e = 0; e < 2 * D; ++e) r[e] = row[e];
-  unsigned h = 1u;
-#pragma unroll
-  for (int d = 0; d < D; ++d) h &= static_cast<unsigned>(may_hit(r[2 * d], r[2 * d + 1], lo[d], hi[d]));
-  return __ballot(h != 0u);
-}
-
-// The (L, I) rows of the J taken points of a tile, loaded together
-// (wave-uniform addresses: scalar loads).
-template <int D, int J, typename T>
-__device__ __forceinline__ void load_rows(const T* __restrict__ tile, const int (&jj)[J],
-                                          T (&li)[J][2 * D]) {
-#pragma unroll
-  for (int q = 0; q < J; ++q)
-#pragma unroll
-    for (int e = 0; e < 2 * D; ++e) li[q][e] = tile[jj[q] * D * 2 + e];
-}
-
-// Takes up to J points from the hit mask (wave-uniform). Missing slots
-// repeat slot 0 and are flagged unused; mask = union of the taken points'
-// non-constant dimensions; w = the points' weights W[j0 + jj] (0 for unused
-// slots; none without kW), loaded before the products so that their latency
-// overlaps the evaluation.
-template <int J, bool kW, typename T>
-__device__ __forceinline__ void take_hits(unsigned long long& bal, const uint32_t* __restrict__ pm,
-                                          const T* __restrict__ W, int j0, int (&jj)[J],
-                                          bool (&used)[J], T (&w)[J], uint32_t& mask) {
-  mask = 0u;
-#pragma unroll
-  for (int q = 0; q < J; ++q) {
-    if (bal) {
-      const int b = __builtin_ctzll(bal);
-      bal &= bal - 1;
-      jj[q] = b;
-      used[q] = true;
-      mask |= pm[j0 + b];
-      w[q] = kW ? W[j0 + b] : T(0);
-    } else {
-      jj[q] = jj[0];
-      used[q] = false;
-      w[q] = T(0);
-    }
-  }
-}
-
 // ---------------------------------------------------------------- mult
 // Lanes own data points (K per lane, coordinates in registers). The data are
 // stored in Morton order, so a wave's 64*K points fill a small box. For every
@@ -276,25 +106,26 @@ __device__ __forceinline__ void take_hits(unsigned long long& bal, const uint32_
 //      over the union of their non-constant dimensions.
 // Every dropped term is exactly zero, so the sums equal a full evaluation.
 // X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] tile boxes,
-// A: [Npad] alpha*scale (zero padded), pm: [Npad] per-point dimension masks,
+// A: [Npad] alpha*scale (zero padded),
 // part: [G][ldx]. Block: kBlock*K data points x grid chunk [y*chunk, +chunk).
 template <int D, int K, int J, typename T>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
-                                                      const T* __restrict__ TB,
+                                                      const float* __restrict__ TB,
                                                       const T* __restrict__ A,
-                                                      const uint32_t* __restrict__ pm,
                                                       double* __restrict__ part, int chunk,
                                                       int jend_all) {
-  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + threadIdx.x;
+  // a wave's 64*K points are contiguous (a small box in Morton order): point
+  // k of a lane is m0 + 64*k
   const int lane = threadIdx.x & 63;
+  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + (threadIdx.x >> 6) * (64 * K) + lane;
   T x[K][D];
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * kBlock];
-  T lo[D], hi[D];
-  wave_box<D, K, T>(x, lo, hi);
+    for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * 64];
+  WaveBox<D, T> box;
+  wave_box<D, D, K, T>(x, box);
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
@@ -302,9 +133,9 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
   const int je = min(jb + chunk, jend_all);
   for (int j0 = jb; j0 < je; j0 += kTG) {
     const int64_t t = j0 / kTG;
-    if (!tile_may_hit<D, T>(TB, t, lo, hi)) continue;
+    if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
     const T* tile = LI + t * D * kTG * 2;
-    unsigned long long bal = tile_hits<D, T>(tile, lane, lo, hi);
+    unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
@@ -312,23 +143,9 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
       int jj[J];
       bool used[J];
       T a[J];
-      uint32_t mask;
-      take_hits<J, true, T>(bal, pm, A, j0, jj, used, a, mask);
-      T li[J][2 * D];
-      load_rows<D, J, T>(tile, jj, li);
+      take_hits<J, true, T>(bal, A, j0, jj, used, a);
       T p[J][K];
-#pragma unroll
-      for (int q = 0; q < J; ++q)
-#pragma unroll
-        for (int k = 0; k < K; ++k) p[q][k] = T(1);
-#pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (!((mask >> d) & 1u)) continue;
-#pragma unroll
-        for (int q = 0; q < J; ++q)
-#pragma unroll
-          for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[q][2 * d], x[k][d], li[q][2 * d + 1]);
-      }
+      eval_group<D, J, K, T>(tile, jj, x, p);
 #pragma unroll
       for (int q = 0; q < J; ++q)
 #pragma unroll
@@ -339,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
   }
   double* out = part + static_cast<int64_t>(blockIdx.y) * ldx;
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[m0 + k * kBlock] = accd[k];
+  for (int k = 0; k < K; ++k) out[m0 + k * 64] = accd[k];
 }
 
 // Generic-dimension mult (D > kMaxTemplD): one data point per lane,
@@ -382,8 +199,7 @@ template <int D, int K, int J, typename T>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
-                                                       const T* __restrict__ TB,
-                                                       const uint32_t* __restrict__ pm,
+                                                       const float* __restrict__ TB,
                                                        double* __restrict__ part, int64_t ldp,
                                                        int jend_all, int64_t seg, int64_t mend) {
   __shared__ double accs[kBlock / 64][kChunkT];
@@ -409,35 +225,20 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
       for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m];
       y[k] = (m < we) ? Y[m] : T(0);
     }
-    T lo[D], hi[D];
-    wave_box<D, K, T>(x, lo, hi);
+    WaveBox<D, T> box;
+    wave_box<D, D, K, T>(x, box);
     for (int j0 = jb; j0 < je; j0 += kTG) {
       const int64_t t = j0 / kTG;
-      if (!tile_may_hit<D, T>(TB, t, lo, hi)) continue;
+      if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
       const T* tile = LI + t * D * kTG * 2;
-      unsigned long long bal = tile_hits<D, T>(tile, lane, lo, hi);
+      unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
       while (bal) {
         int jj[J];
         bool used[J];
         T unused_w[J];
-        uint32_t mask;
-        take_hits<J, false, T>(bal, pm, static_cast<const T*>(nullptr), j0, jj, used, unused_w, mask);
-        T li[J][2 * D];
-        load_rows<D, J, T>(tile, jj, li);
+        take_hits<J, false, T>(bal, static_cast<const T*>(nullptr), j0, jj, used, unused_w);
         T p[J][K];
-#pragma unroll
-        for (int q = 0; q < J; ++q)
-#pragma unroll
-          for (int k = 0; k < K; ++k) p[q][k] = T(1);
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          if (!((mask >> d) & 1u)) continue;
-#pragma unroll
-          for (int q = 0; q < J; ++q)
-#pragma unroll
-            for (int k = 0; k < K; ++k)
-              p[q][k] = hat_step(p[q][k], li[q][2 * d], x[k][d], li[q][2 * d + 1]);
-        }
+        eval_group<D, J, K, T>(tile, jj, x, p);
 #pragma unroll
         for (int q = 0; q < J; ++q) {
           if (!used[q]) continue;
@@ -527,33 +328,33 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 
 // ------------------------------------------------------------ dispatch
 template <typename T, int D>
-void launch_mult_D(const T* X, int64_t ldx, const T* LI, const T* TB, const T* A,
-                   const uint32_t* pm, double* part, int chunk, int jend, int G, int64_t Mpad,
+void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const T* A,
+                   double* part, int chunk, int jend, int G, int64_t Mpad,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
   hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, LI,
-                     TB, A, pm, part, chunk, jend);
+                     TB, A, part, chunk, jend);
 }
 
 template <typename T, int D>
-void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const T* TB,
-                    const uint32_t* pm, double* part, int64_t ldp, int jend, int64_t seg,
+void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const float* TB,
+                    double* part, int64_t ldp, int jend, int64_t seg,
                     int64_t mend, int S, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
   dim3 grid(static_cast<unsigned>((jend + kChunkT - 1) / kChunkT), static_cast<unsigned>(S));
   clear_launch_error();
   hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, Y,
-                     LI, TB, pm, part, ldp, jend, seg, mend);
+                     LI, TB, part, ldp, jend, seg, mend);
 }
 
 template <typename T>
-using MultFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const uint32_t*, double*,
-                        int, int, int, int64_t, hipStream_t);
+using MultFn = void (*)(const T*, int64_t, const T*, const float*, const T*, double*, int, int,
+                        int, int64_t, hipStream_t);
 template <typename T>
-using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const uint32_t*, double*,
-                         int64_t, int, int64_t, int64_t, int, hipStream_t);
+using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, double*, int64_t,
+                         int, int64_t, int64_t, int, hipStream_t);
 
 template <typename T, int... Ds>
 struct Table {
@@ -661,6 +462,9 @@ std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d) 
   return tb;
 }
 
+template std::vector<float> tile_boxes<float>(const std::vector<double>&, size_t, size_t);
+template std::vector<double> tile_boxes<double>(const std::vector<double>&, size_t, size_t);
+
 // =================================================================== runtime
 
 void check_hip(hipError_t e, const char* what) {
@@ -753,7 +557,7 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, pmask_, dperm_, A_, Y_,
+  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, dperm_, A_, Y_,
                   part_, hin_, hout_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -807,17 +611,12 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
   // padded points: (L, I) = (0, 0) -> factor 1, weight 0 through A / ignored rows
   li.resize(Npad * D_ * 2, 0.0);
   sc.resize(Npad, 1.0);
-  std::vector<uint32_t> pm(Npad, 0u);
-  for (size_t s = 0; s < N; ++s) pm[s] = pmask[order[s]];
   check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   N_ = N;
   Npad_ = Npad;
   ensure(reinterpret_cast<void**>(&scale_), &gridcap_scale_, Npad_ * sizeof(double));
   check_hip(hipMemcpy(scale_, sc.data(), Npad_ * sizeof(double), hipMemcpyHostToDevice),
             "upload scale");
-  ensure(&pmask_, &pmask_cap_, pm.size() * sizeof(uint32_t));
-  check_hip(hipMemcpy(pmask_, pm.data(), pm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
-            "upload point masks");
   ensure(&perm_, &perm_cap_, N_ * sizeof(int32_t));
   check_hip(hipMemcpy(perm_, order.data(), N_ * sizeof(int32_t), hipMemcpyHostToDevice),
             "upload permutation");
@@ -825,9 +624,10 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
     ensure(&LI64_, &gridcap_li64_, li.size() * sizeof(double));
     check_hip(hipMemcpy(LI64_, li.data(), li.size() * sizeof(double), hipMemcpyHostToDevice),
               "upload LI64");
-    const std::vector<double> tb = tile_boxes<double>(li, Npad_, D_);
-    ensure(&TB64_, &tbcap64_, tb.size() * sizeof(double));
-    check_hip(hipMemcpy(TB64_, tb.data(), tb.size() * sizeof(double), hipMemcpyHostToDevice),
+    // FP32 boxes of the FP64 supports, rounded outwards
+    const std::vector<float> tb = tile_boxes<float>(li, Npad_, D_);
+    ensure(&TB64_, &tbcap64_, tb.size() * sizeof(float));
+    check_hip(hipMemcpy(TB64_, tb.data(), tb.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload tile boxes");
   }
   if (serves(Precision::FP32)) {
@@ -848,7 +648,7 @@ void DeviceMultiEval::record_start() { check_hip(hipEventRecord(ev0_, stream_), 
 void DeviceMultiEval::record_stop() { check_hip(hipEventRecord(ev1_, stream_), "hipEventRecord"); }
 
 template <typename T>
-void DeviceMultiEval::run_mult(const T* X, const T* LI, const T* TB, const double* d_alpha,
+void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const double* d_alpha,
                                double* d_result, hipEvent_t* tev) {
   hipStream_t s = stream_;
   ensure(&A_, &A_cap_, Npad_ * sizeof(T));
@@ -882,7 +682,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const T* TB, const doubl
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
     MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_));
     f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A_),
-      static_cast<const uint32_t*>(pmask_), P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
+      P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();
@@ -902,7 +702,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const T* TB, const doubl
 }
 
 template <typename T>
-void DeviceMultiEval::run_multT(const T* X, const T* LI, const T* TB, const double* d_y,
+void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const double* d_y,
                                 double* d_result, hipEvent_t* tev) {
   hipStream_t s = stream_;
   ensure(&Y_, &Y_cap_, Mpad_ * sizeof(T));
@@ -940,7 +740,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const T* TB, const doub
   if (templ) {
     MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_));
     f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
-      static_cast<const uint32_t*>(pmask_), P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
+      P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
       s);
   } else {
     dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(S));
@@ -969,7 +769,7 @@ void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Preci
   record_start();
   if (p == Precision::FP64)
     run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
-                     static_cast<const double*>(TB64_), d_alpha, d_result, tp ? tev : nullptr);
+                     static_cast<const float*>(TB64_), d_alpha, d_result, tp ? tev : nullptr);
   else
     run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
                     static_cast<const float*>(TB32_), d_alpha, d_result, tp ? tev : nullptr);
@@ -985,7 +785,7 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
   record_start();
   if (p == Precision::FP64)
     run_multT<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
-                      static_cast<const double*>(TB64_), d_y, d_result, tp ? tev : nullptr);
+                      static_cast<const float*>(TB64_), d_y, d_result, tp ? tev : nullptr);
   else
     run_multT<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
                      static_cast<const float*>(TB32_), d_y, d_result, tp ? tev : nullptr);

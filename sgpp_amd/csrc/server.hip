@@ -6,8 +6,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 
+#include "cull_device.hpp"
 #include "server.hpp"
 
 namespace sgpp_hip {
@@ -34,7 +38,7 @@ struct Item {
   int nq;          // valid queries in the tile
   int j0, j1;      // grid chunk, multiples of kTG
   long long xoff;  // element offset of the request's query block [D][ldq]
-  int ldq;         // padded query count of the request (multiple of the tile)
+  int ldq;         // padded query count of the block (multiple of the tile)
   int pad;
   long long poff;  // double offset of this item's partial row (tile doubles)
 };
@@ -60,17 +64,6 @@ constexpr int serve_j(int D) {
   return (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D) < 8 ? (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D)
                                                           : 8;
 }
-template <typename T>
-int serve_k_rt(int D) {
-  return sizeof(T) == 8 ? (D <= 5 ? 4 : (D <= 10 ? 2 : 1)) : (D <= 4 ? 8 : (D <= 10 ? 4 : 2));
-}
-
-__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
-__device__ __forceinline__ float absT(float a) { return __builtin_fabsf(a); }
-__device__ __forceinline__ double absT(double a) { return __builtin_fabs(a); }
-__device__ __forceinline__ float clamp01(float v) { return __builtin_amdgcn_fmed3f(v, 0.0f, 1.0f); }
-__device__ __forceinline__ double clamp01(double v) { return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0); }
 
 // one work item: a tile of kBlock*K queries of one request against one grid
 // chunk. The grid tile is walked in sub-tiles of J points; per (sub-tile,
@@ -142,6 +135,10 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
   for (int k = 0; k < K; ++k) out[k * kBlock + threadIdx.x] = accd[k];
 }
 
+int serve_k_rt(Precision p, int D) {
+  return p == Precision::FP64 ? serve_k<double>(D) : serve_k<float>(D);
+}
+
 template <typename T, int D>
 __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md, const T* X,
                                              double* part, T* sLI, T* sA, uint32_t* sM) {
@@ -205,6 +202,51 @@ __global__ void reset_counter(unsigned* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
 }
 
+// Work items and reduction items of one block of q queries of one model
+// (stored [D][ldq] at xoff): ntiles query tiles x nchunks grid chunks;
+// partial rows start at part_base, results go to out_off.
+void add_items(int table_index, int D, size_t Npad, size_t tile, size_t chunk, size_t q,
+               size_t ldq, long long xoff, long long out_off, size_t part_base, Item* items,
+               RedItem* red) {
+  const size_t ntiles = ldq / tile;
+  const size_t nchunks = (Npad + chunk - 1) / chunk;
+  for (size_t tI = 0; tI < ntiles; ++tI) {
+    RedItem& ri = red[tI];
+    ri.pbase = static_cast<long long>(part_base + tI * nchunks * tile);
+    ri.nchunks = static_cast<int>(nchunks);
+    ri.nq = static_cast<int>(std::min<size_t>(tile, q - std::min(q, tI * tile)));
+    ri.tile = static_cast<int>(tile);
+    ri.pad = 0;
+    ri.out = out_off + static_cast<long long>(tI * tile);
+    for (size_t c = 0; c < nchunks; ++c) {
+      Item& it = items[tI * nchunks + c];
+      it.model = table_index;
+      it.D = D;
+      it.q0 = static_cast<int>(tI * tile);
+      it.nq = ri.nq;
+      it.j0 = static_cast<int>(c * chunk);
+      it.j1 = static_cast<int>(std::min<size_t>((c + 1) * chunk, Npad));
+      it.xoff = xoff;
+      it.ldq = static_cast<int>(ldq);
+      it.pad = 0;
+      it.poff = ri.pbase + static_cast<long long>(c * tile);
+    }
+  }
+}
+
+uint64_t splitmix64(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// autotuner: synthetic request size, candidate grid chunks, partial cap
+constexpr size_t kTuneQueries = 16384;
+constexpr size_t kTuneBatchCopies = 8;
+constexpr int kTuneChunks[] = {256, 512, 1024, 2048, 4096, 8192, 16384, 32768};
+constexpr size_t kTunePartDoubles = size_t(1) << 25;
+
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
       .count();
@@ -220,11 +262,15 @@ struct Server::Model {
   size_t N = 0, Npad = 0;
   void* LI = nullptr;
   void* A = nullptr;
-  void* tmask = nullptr;
+  void* pm = nullptr;  // [Npad/4] dimensions evaluated per 4 consecutive points
   size_t bytes = 0;
-  int tile = 0;    // queries per work item
-  int chunk = 0;   // grid points per work item
   int active = 0;  // requests queued or running
+  void free_device() {
+    void* p[] = {LI, A, pm};
+    for (void* q : p)
+      if (q) (void)hipFree(q);
+    LI = A = pm = nullptr;
+  }
 };
 
 struct Server::Request {
@@ -232,7 +278,8 @@ struct Server::Request {
   int model = 0;
   size_t q = 0;
   size_t ldq = 0;
-  std::vector<char> X;  // [D][ldq] in the compute type, zero padded
+  size_t qoff = 0;      // column of the first query in its model's block of the batch
+  std::vector<char> X;  // [D][q] in the compute type
   std::vector<double> out;
   double t_submit = 0.0, t_done = 0.0, deadline = 0.0;
   int status = REQ_QUEUED;
@@ -262,6 +309,12 @@ struct Server::Slot {
   bool busy = false;
   bool inject = false;
   std::vector<std::shared_ptr<Request>> reqs;
+  struct Block {
+    long long xoff;
+    int D;
+    size_t q, ldq;
+  };
+  std::vector<Block> blocks;  // one per model in the batch (padding columns)
   size_t usedX = 0, nItems = 0, nRed = 0, usedPart = 0, usedOut = 0;
 };
 
@@ -337,6 +390,7 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     }
     slots_.push_back(std::move(s));
   }
+  load_cache();
   thread_ = std::thread(&Server::loop, this);
 }
 
@@ -363,11 +417,7 @@ Server::~Server() {
     (void)hipEventDestroy(s->ev_done);
     (void)hipStreamDestroy(s->stream);
   }
-  for (auto& kv : models_) {
-    if (kv.second->LI) (void)hipFree(kv.second->LI);
-    if (kv.second->A) (void)hipFree(kv.second->A);
-    if (kv.second->tmask) (void)hipFree(kv.second->tmask);
-  }
+  for (auto& kv : models_) kv.second->free_device();
   if (model_table_) (void)hipFree(model_table_);
 }
 
@@ -407,12 +457,6 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
   m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + (m->Npad / 4) * sizeof(uint32_t);
-  const int K = f64 ? serve_k_rt<double>(m->D) : serve_k_rt<float>(m->D);
-  m->tile = kBlock * K;
-  const double per_point = static_cast<double>(m->tile) * (2 * dim + 1);
-  size_t chunk = static_cast<size_t>(std::ceil(cfg_.item_work / per_point / kTG)) * kTG;
-  chunk = std::max<size_t>(kTG, std::min(chunk, m->Npad));
-  m->chunk = static_cast<int>(chunk);
   std::vector<double> A(m->Npad, 0.0);
   for (size_t s2 = 0; s2 < N; ++s2) A[s2] = alpha[order[s2]] * sc[s2];
   // tile-major (L, I); padded points (0, 0) evaluate to 1 with weight 0
@@ -451,8 +495,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
     check_hip(hipMalloc(&m->LI, m->Npad * dim * 2 * esz), "hipMalloc(model LI)");
     check_hip(hipMalloc(&m->A, m->Npad * esz), "hipMalloc(model A)");
-    check_hip(hipMalloc(&m->tmask, qm.size() * sizeof(uint32_t)), "hipMalloc(model masks)");
-    check_hip(hipMemcpy(m->tmask, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+    check_hip(hipMalloc(&m->pm, qm.size() * sizeof(uint32_t)), "hipMalloc(model masks)");
+    check_hip(hipMemcpy(m->pm, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
               "upload masks");
     if (f64) {
       check_hip(hipMemcpy(m->LI, li.data(), li.size() * 8, hipMemcpyHostToDevice), "upload LI");
@@ -462,7 +506,7 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       check_hip(hipMemcpy(m->LI, l32.data(), l32.size() * 4, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, a32.data(), a32.size() * 4, hipMemcpyHostToDevice), "upload A");
     }
-    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->tmask), m->D,
+    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->pm), m->D,
                  static_cast<int>(m->Npad)};
     check_hip(hipMemcpy(static_cast<ModelDesc*>(model_table_) + m->table_index, &md,
                         sizeof(ModelDesc), hipMemcpyHostToDevice),
@@ -471,13 +515,34 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     std::lock_guard<std::mutex> lk(mu_);
     models_.erase(m->id);
     used_bytes_ -= m->bytes;
-    if (m->LI) (void)hipFree(m->LI);
-    if (m->A) (void)hipFree(m->A);
-    if (m->tmask) (void)hipFree(m->tmask);
+    m->free_device();
     throw;
+  }
+  if (cfg_.autotune) {
+    bool have;
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      have = tuned_.count(tune_key(basis, m->D)) > 0;
+    }
+    if (!have) {
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        m->active++;  // not evictable while its launches run
+      }
+      try {
+        tune(*m);
+      } catch (...) {
+        std::lock_guard<std::mutex> lk(mu_);
+        m->active--;
+        throw;
+      }
+      std::lock_guard<std::mutex> lk(mu_);
+      m->active--;
+    }
   }
   return m->id;
 }
+
 
 void Server::evict_model(int id) {
   std::shared_ptr<Model> m;
@@ -490,9 +555,7 @@ void Server::evict_model(int id) {
     models_.erase(it);
     used_bytes_ -= m->bytes;
   }
-  check_hip(hipFree(m->LI), "hipFree");
-  check_hip(hipFree(m->A), "hipFree");
-  check_hip(hipFree(m->tmask), "hipFree");
+  m->free_device();
 }
 
 size_t Server::model_bytes(int id) {
@@ -509,29 +572,30 @@ size_t Server::memory_used() {
 
 uint64_t Server::submit(int model, size_t q, const double* queries, double deadline_s) {
   std::shared_ptr<Model> m;
+  if (q == 0) throw input_error("empty request");
   {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = models_.find(model);
     if (it == models_.end()) throw input_error("unknown model");
     m = it->second;
   }
-  if (q == 0) throw input_error("empty request");
   const double t_enter = now_s();
   auto r = std::make_shared<Request>();
   r->model = model;
   r->q = q;
-  r->ldq = ((q + m->tile - 1) / m->tile) * m->tile;
   const size_t D = m->D;
+  for (size_t i = 0; i < q * D; ++i)
+    if (!(queries[i] >= 0.0 && queries[i] <= 1.0))
+      throw input_error("query points must lie in [0,1]^d");
   const bool f64 = cfg_.precision == Precision::FP64;
-  r->X.assign(D * r->ldq * (f64 ? 8 : 4), 0);
+  r->X.resize(D * q * (f64 ? 8 : 4));
   for (size_t i = 0; i < q; ++i)
     for (size_t d = 0; d < D; ++d) {
       const double v = queries[i * D + d];
-      if (!(v >= 0.0 && v <= 1.0)) throw input_error("query points must lie in [0,1]^d");
       if (f64)
-        reinterpret_cast<double*>(r->X.data())[d * r->ldq + i] = v;
+        reinterpret_cast<double*>(r->X.data())[d * q + i] = v;
       else
-        reinterpret_cast<float*>(r->X.data())[d * r->ldq + i] = static_cast<float>(v);
+        reinterpret_cast<float*>(r->X.data())[d * q + i] = static_cast<float>(v);
     }
   r->t_submit = t_enter;  // latency includes the conversion of the queries
   r->deadline = deadline_s > 0 ? r->t_submit + deadline_s : 0.0;
@@ -599,10 +663,32 @@ void Server::fail_request(const std::shared_ptr<Request>& r, int status, const c
 }
 
 // caller holds mu_: move requests from the queue into slot s (FIFO)
+// caller holds mu_: move requests from the queue into slot s (FIFO). The
+// queries of all requests for one model form one block [D][ldq] of the batch
+// (in queue order), tiled and chunked together: a query's result depends only
+// on the model and its grid chunks, so packing changes no result bit, and a
+// request pays for at most one partially filled tile per model and batch.
 bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
   s.usedX = s.nItems = s.nRed = s.usedPart = s.usedOut = 0;
+  s.blocks.clear();
   const double t = now_s();
-  const size_t esz = cfg_.precision == Precision::FP64 ? 8 : 4;
+  struct Group {
+    const Model* m;
+    size_t tile, chunk, nchunks, q;
+    std::vector<std::shared_ptr<Request>> reqs;
+  };
+  std::vector<Group> groups;
+  std::map<int, size_t> gidx;
+  // resources of a block of q queries
+  auto need = [](const Group& g, size_t q, size_t* xel, size_t* items, size_t* part,
+                 size_t* red) {
+    const size_t ntiles = (q + g.tile - 1) / g.tile;
+    *xel = static_cast<size_t>(g.m->D) * ntiles * g.tile;
+    *items = ntiles * g.nchunks;
+    *part = *items * g.tile;
+    *red = ntiles;
+  };
+  size_t useX = 0, useItems = 0, usePart = 0, useRed = 0, useOut = 0;
   while (!queue_.empty() && taken.size() < static_cast<size_t>(cfg_.max_requests_per_batch)) {
     std::shared_ptr<Request> r = queue_.front();
     if (r->status != REQ_QUEUED) {  // cancelled meanwhile
@@ -615,58 +701,76 @@ bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
       continue;
     }
     const Model& m = *models_.at(r->model);
-    const size_t ntiles = r->ldq / m.tile;
-    const size_t nchunks = (m.Npad + m.chunk - 1) / m.chunk;
-    const size_t items = ntiles * nchunks;
-    const size_t part = items * m.tile;
-    const size_t xel = static_cast<size_t>(m.D) * r->ldq;
-    if (items > s.capItems || part > s.capPart || xel > s.capX || r->q > s.capOut ||
-        ntiles > s.capRed) {
+    Group probe{&m, static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D), chunk_for(m), 0,
+                0, {}};
+    probe.nchunks = (m.Npad + probe.chunk - 1) / probe.chunk;
+    size_t x1, i1, p1, r1;
+    need(probe, r->q, &x1, &i1, &p1, &r1);
+    if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX || r->q > s.capOut || r1 > s.capRed) {
       queue_.pop_front();
       fail_request(r, REQ_FAILED_BUDGET, "request exceeds the batch buffer capacity");
       continue;
     }
-    if (s.nItems + items > s.capItems || s.usedPart + part > s.capPart ||
-        s.usedX + xel > s.capX || s.usedOut + r->q > s.capOut || s.nRed + ntiles > s.capRed)
+    auto gi = gidx.find(r->model);
+    const Group& g = gi == gidx.end() ? probe : groups[gi->second];
+    size_t x0 = 0, i0 = 0, p0 = 0, r0 = 0;
+    if (gi != gidx.end()) need(g, g.q, &x0, &i0, &p0, &r0);
+    need(g, g.q + r->q, &x1, &i1, &p1, &r1);
+    if (useX + x1 - x0 > s.capX || useItems + i1 - i0 > s.capItems ||
+        usePart + p1 - p0 > s.capPart || useRed + r1 - r0 > s.capRed ||
+        useOut + r->q > s.capOut)
       break;  // full: the rest waits for the next slot (FIFO)
+    useX += x1 - x0;
+    useItems += i1 - i0;
+    usePart += p1 - p0;
+    useRed += r1 - r0;
+    useOut += r->q;
     queue_.pop_front();
     r->status = REQ_RUNNING;
-    r->xoff = static_cast<long long>(s.usedX);
-    r->out_off = static_cast<long long>(s.usedOut);
-    for (size_t tI = 0; tI < ntiles; ++tI) {
-      RedItem& ri = s.hRed[s.nRed++];
-      ri.pbase = static_cast<long long>(s.usedPart + tI * nchunks * m.tile);
-      ri.nchunks = static_cast<int>(nchunks);
-      ri.nq = static_cast<int>(std::min<size_t>(m.tile, r->q - tI * m.tile));
-      ri.tile = m.tile;
-      ri.pad = 0;
-      ri.out = r->out_off + static_cast<long long>(tI * m.tile);
-      for (size_t c = 0; c < nchunks; ++c) {
-        Item& it = s.hItems[s.nItems++];
-        it.model = m.table_index;
-        it.D = m.D;
-        it.q0 = static_cast<int>(tI * m.tile);
-        it.nq = ri.nq;
-        it.j0 = static_cast<int>(c * m.chunk);
-        it.j1 = static_cast<int>(std::min<size_t>((c + 1) * m.chunk, m.Npad));
-        it.xoff = r->xoff;
-        it.ldq = static_cast<int>(r->ldq);
-        it.pad = 0;
-        it.poff = ri.pbase + static_cast<long long>(c * m.tile);
-      }
+    if (gi == gidx.end()) {
+      gidx[r->model] = groups.size();
+      groups.push_back(probe);
     }
-    s.usedPart += part;
-    s.usedX += xel;
-    s.usedOut += r->q;
-    (void)esz;
+    Group& gg = groups[gidx[r->model]];
+    r->qoff = gg.q;
+    gg.q += r->q;
+    gg.reqs.push_back(r);
     taken.push_back(r);
+  }
+  for (Group& g : groups) {
+    const size_t ntiles = (g.q + g.tile - 1) / g.tile;
+    const size_t ldq = ntiles * g.tile;
+    const long long xoff = static_cast<long long>(s.usedX);
+    const long long out0 = static_cast<long long>(s.usedOut);
+    for (auto& r : g.reqs) {
+      r->xoff = xoff;
+      r->ldq = ldq;
+      r->out_off = out0 + static_cast<long long>(r->qoff);
+    }
+    add_items(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q, ldq, xoff, out0,
+              s.usedPart, s.hItems + s.nItems, s.hRed + s.nRed);
+    s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq});
+    s.nItems += ntiles * g.nchunks;
+    s.nRed += ntiles;
+    s.usedPart += ntiles * g.nchunks * g.tile;
+    s.usedX += static_cast<size_t>(g.m->D) * ldq;
+    s.usedOut += g.q;
   }
   return !taken.empty();
 }
 
 void Server::launch_batch(Slot& s) {
   const size_t esz = cfg_.precision == Precision::FP64 ? 8 : 4;
-  for (auto& r : s.reqs) std::memcpy(s.hX + r->xoff * esz, r->X.data(), r->X.size());
+  // each request's rows into its model's block, padding columns zero
+  for (auto& r : s.reqs) {
+    const size_t D = r->X.size() / (r->q * esz);
+    for (size_t d = 0; d < D; ++d)
+      std::memcpy(s.hX + (r->xoff + d * r->ldq + r->qoff) * esz, r->X.data() + d * r->q * esz,
+                  r->q * esz);
+  }
+  for (const Slot::Block& b : s.blocks)
+    for (int d = 0; d < b.D; ++d)
+      std::memset(s.hX + (b.xoff + d * b.ldq + b.q) * esz, 0, (b.ldq - b.q) * esz);
   s.hCounts[0] = static_cast<int>(s.nItems);
   s.hCounts[1] = static_cast<int>(s.nRed);
   s.hCounts[2] = s.hCounts[3] = 0;
@@ -732,6 +836,189 @@ void Server::finish_batch(Slot& s) {
   s.inject = false;
   running_--;
   done_cv_.notify_all();
+}
+
+int Server::tune_key(Basis basis, int D) { return (basis == Basis::ModLinear ? 64 : 0) + D; }
+
+// caller holds mu_: grid points per work item of model m
+size_t Server::chunk_for(const Model& m) const {
+  size_t chunk;
+  auto it = tuned_.find(tune_key(m.basis, m.D));
+  if (it != tuned_.end()) {
+    chunk = static_cast<size_t>(it->second.chunk);
+  } else {
+    const double per_point =
+        static_cast<double>(kBlock) * serve_k_rt(cfg_.precision, m.D) * (2 * m.D + 1);
+    chunk = static_cast<size_t>(std::ceil(cfg_.item_work / per_point / kTG)) * kTG;
+  }
+  return std::max<size_t>(kTG, std::min(chunk, m.Npad));
+}
+
+bool Server::tuned_shape(Basis basis, int dim, TunedShape* out) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = tuned_.find(tune_key(basis, dim));
+  if (it == tuned_.end()) return false;
+  if (out) *out = it->second;
+  return true;
+}
+
+// Times every candidate grid chunk (grid points per work item) on the model
+// with synthetic uniform queries, alone on its own stream, and keeps the
+// fastest with its cost per (query x grid point). The load follows the
+// server's batching: one request of kTuneQueries when every launch holds one
+// request (latency of a lone request), kTuneBatchCopies times as many when
+// batches pack many requests (throughput of a saturated device). The result
+// applies to every model of the same basis and dimension on this server and
+// is written to the cache file when one is configured.
+void Server::tune(Model& m) {
+  const bool f64 = cfg_.precision == Precision::FP64;
+  const size_t esz = f64 ? 8 : 4;
+  const size_t D = static_cast<size_t>(m.D);
+  const size_t nq = kTuneQueries * (cfg_.max_requests_per_batch == 1 ? 1 : kTuneBatchCopies);
+  const size_t tile = static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D);
+  const size_t ntiles = nq / tile;
+  std::vector<char> hx(nq * D * esz);
+  uint64_t st = 0x5EED0000ull + static_cast<uint64_t>(D);
+  for (size_t d = 0; d < D; ++d)
+    for (size_t i = 0; i < nq; ++i) {
+      const double v = static_cast<double>(splitmix64(st) >> 11) * 0x1.0p-53;
+      if (f64)
+        reinterpret_cast<double*>(hx.data())[d * nq + i] = v;
+      else
+        reinterpret_cast<float*>(hx.data())[d * nq + i] = static_cast<float>(v);
+    }
+  std::vector<int> cands;
+  for (int c : kTuneChunks)
+    if (static_cast<size_t>(c) < m.Npad && ntiles * ((m.Npad + c - 1) / c) * tile <= kTunePartDoubles)
+      cands.push_back(c);
+  cands.push_back(static_cast<int>(m.Npad));
+  size_t max_items = 0;
+  for (int c : cands) max_items = std::max(max_items, ntiles * ((m.Npad + c - 1) / c));
+  check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void* dX = nullptr;
+  Item* dItems = nullptr;
+  int* dCounts = nullptr;
+  unsigned* dCounter = nullptr;
+  double* dPart = nullptr;
+  auto release = [&] {
+    if (stream) (void)hipStreamSynchronize(stream);
+    void* p[] = {dX, dItems, dCounts, dCounter, dPart};
+    for (void* x : p)
+      if (x) (void)hipFree(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+  };
+  TunedShape res{};
+  try {
+    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    check_hip(hipMalloc(&dX, hx.size()), "hipMalloc(tune X)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dItems), max_items * sizeof(Item)),
+              "hipMalloc(tune items)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), 4 * sizeof(int)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), sizeof(unsigned)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dPart),
+                        std::max(kTunePartDoubles, ntiles * tile) * sizeof(double)),
+              "hipMalloc(tune partials)");
+    check_hip(hipMemcpy(dX, hx.data(), hx.size(), hipMemcpyHostToDevice), "upload tune X");
+    double best = -1.0;
+    for (int c : cands) {
+      const size_t nchunks = (m.Npad + c - 1) / c;
+      const size_t nitems = ntiles * nchunks;
+      std::vector<Item> items(nitems);
+      std::vector<RedItem> red(ntiles);
+      add_items(m.table_index, m.D, m.Npad, tile, static_cast<size_t>(c), nq, nq, 0, 0, 0,
+                items.data(), red.data());
+      const int counts[4] = {static_cast<int>(nitems), 0, 0, 0};
+      check_hip(hipMemcpy(dItems, items.data(), nitems * sizeof(Item), hipMemcpyHostToDevice),
+                "upload tune items");
+      check_hip(hipMemcpy(dCounts, counts, sizeof(counts), hipMemcpyHostToDevice),
+                "upload tune counts");
+      float tmin = -1.0f;
+      for (int rep = 0; rep < 4; ++rep) {  // the first run warms up
+        clear_launch_error();
+        check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, dCounter);
+        if (f64)
+          hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
+                             static_cast<const ModelDesc*>(model_table_), dItems, dCounts,
+                             dCounter, static_cast<const double*>(dX), dPart);
+        else
+          hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
+                             static_cast<const ModelDesc*>(model_table_), dItems, dCounts,
+                             dCounter, static_cast<const float*>(dX), dPart);
+        check_hip(hipGetLastError(), "tune launch");
+        check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (rep > 0 && (tmin < 0.0f || ms < tmin)) tmin = ms;
+      }
+      const double cost = tmin * 1e-3 / (static_cast<double>(nq) * static_cast<double>(m.Npad));
+      if (best < 0.0 || cost < best) {
+        best = cost;
+        res.chunk = c;
+        res.cost = cost;
+      }
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
+  std::lock_guard<std::mutex> lk(mu_);
+  tuned_[tune_key(m.basis, m.D)] = res;
+  save_cache();
+}
+
+// Cache file: one line per tuned shape,
+//   sgpp_hip_server_shape <precision> <basis> <dim> <chunk> <cost>
+// (precision 0 = FP64, 1 = FP32; basis 0 = Linear, 1 = ModLinear).
+void Server::load_cache() {
+  if (cfg_.autotune_cache.empty()) return;
+  std::ifstream f(cfg_.autotune_cache);
+  std::string tag;
+  int prec, basis, dim;
+  TunedShape t;
+  const int my_prec = cfg_.precision == Precision::FP64 ? 0 : 1;
+  while (f >> tag >> prec >> basis >> dim >> t.chunk >> t.cost) {
+    if (tag != "sgpp_hip_server_shape" || prec != my_prec || dim < 1 || dim > kMaxD) continue;
+    if (t.chunk < kTG || t.chunk % kTG) continue;
+    tuned_[tune_key(basis ? Basis::ModLinear : Basis::Linear, dim)] = t;
+  }
+}
+
+// caller holds mu_; keeps the other precision's lines
+void Server::save_cache() {
+  if (cfg_.autotune_cache.empty()) return;
+  const int my_prec = cfg_.precision == Precision::FP64 ? 0 : 1;
+  std::vector<std::string> keep;
+  {
+    std::ifstream f(cfg_.autotune_cache);
+    std::string line;
+    while (std::getline(f, line)) {
+      std::istringstream is(line);
+      std::string tag;
+      int prec = -1;
+      is >> tag >> prec;
+      if (tag == "sgpp_hip_server_shape" && prec != my_prec) keep.push_back(line);
+    }
+  }
+  const std::string tmp = cfg_.autotune_cache + ".tmp";
+  {
+    std::ofstream f(tmp);
+    if (!f) return;  // a cache that cannot be written is not an error
+    for (const std::string& l : keep) f << l << "\n";
+    f.precision(9);
+    for (const auto& kv : tuned_)
+      f << "sgpp_hip_server_shape " << my_prec << " " << (kv.first >= 64 ? 1 : 0) << " "
+        << (kv.first % 64) << " " << kv.second.chunk << " " << kv.second.cost << "\n";
+  }
+  (void)std::rename(tmp.c_str(), cfg_.autotune_cache.c_str());
 }
 
 void Server::loop() {

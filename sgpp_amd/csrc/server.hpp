@@ -13,9 +13,18 @@
 //   * a batch is a work list of items (model, query tile, grid chunk) run by
 //     one persistent kernel whose blocks pull items from a counter, followed
 //     by a reduction kernel; both are replayed from a captured hipGraph;
-//   * partials of a request are summed over its grid chunks in chunk order,
-//     and a request's tiling depends only on (q, model), so a result is
-//     bit-identical whether the request runs alone or in any mix.
+//   * the kernel stages 64-point grid tiles in LDS and evaluates them in
+//     sub-tiles, skipping per sub-tile the dimensions in which every point is
+//     constant (ModLinear level 1; models are stored sorted by that set);
+//   * the queries of all requests of a batch for one model form one block,
+//     tiled together (a request pays for at most one partial query tile per
+//     model and batch); the grid chunk per work item comes from an autotuner
+//     that times the candidates per (basis, dimension) on the first admitted
+//     model and can persist them in a cache file;
+//   * partials of a query are summed over its model's grid chunks in chunk
+//     order, and a query's result depends only on its model and that
+//     chunking, so a result is bit-identical whether the request runs alone
+//     or in any mix.
 // Per-request failures (over budget, deadline passed, cancelled, an injected
 // kernel-error flag) are reported on the request and never stop the loop.
 #pragma once
@@ -49,7 +58,16 @@ struct ServerConfig {
   int max_requests_per_batch = 1 << 16;
   int blocks = 0;             // persistent grid size; 0 = 8 per CU
   bool use_graph = true;
-  double item_work = 33554432.0;  // target lane-operations per work item (tile x chunk x (2d+1))
+  double item_work = 33554432.0;  // untuned: lane-operations per work item (tile x chunk x (2d+1))
+  bool autotune = false;          // time tile shapes per (basis, dimension) at admission
+  std::string autotune_cache;     // file of tuned shapes, read at start and updated; "" = none
+};
+
+// Tuned shape of one (basis, dimension): grid points per work item and the
+// measured cost in seconds per (query x grid point).
+struct TunedShape {
+  int chunk = 0;
+  double cost = 0.0;
 };
 
 enum RequestStatus : int {
@@ -95,6 +113,8 @@ class Server {
   bool cancel(uint64_t id);
   // makes the next batch report a device error on all its requests
   void inject_kernel_error();
+  // the tuned shape of (basis, dim), if tuned or loaded from the cache
+  bool tuned_shape(Basis basis, int dim, TunedShape* out);
   ServerStats stats();
   void drain();  // blocks until no request is queued or running
 
@@ -102,6 +122,11 @@ class Server {
   struct Model;
   struct Request;
   struct Slot;
+  size_t chunk_for(const Model& m) const;
+  void tune(Model& m);
+  void load_cache();
+  void save_cache();
+  static int tune_key(Basis basis, int D);
   void loop();
   bool form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken);
   void launch_batch(Slot& s);
@@ -117,6 +142,7 @@ class Server {
   std::deque<std::shared_ptr<Request>> queue_;
   std::map<uint64_t, std::shared_ptr<Request>> requests_;
   std::map<int, std::shared_ptr<Model>> models_;
+  std::map<int, TunedShape> tuned_;  // by tune_key
   std::vector<std::unique_ptr<Slot>> slots_;
   void* model_table_ = nullptr;  // device array of model descriptors
   int next_model_ = 0;

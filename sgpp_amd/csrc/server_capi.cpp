@@ -68,6 +68,8 @@ sgpp_hip_status sgpp_hip_server_create(const sgpp_hip_server_config* c, sgpp_hip
       cfg.blocks = c->blocks;
       cfg.use_graph = c->use_graph != 0;
       if (c->item_work > 0) cfg.item_work = c->item_work;
+      cfg.autotune = c->autotune != 0;
+      if (c->autotune_cache) cfg.autotune_cache = c->autotune_cache;
     }
     auto* h = new sgpp_hip_server();
     try {
@@ -186,6 +188,26 @@ sgpp_hip_status sgpp_hip_server_stats(sgpp_hip_server* h, sgpp_hip_server_statis
     out->batches = st.batches;
     out->kernel_launches = st.kernel_launches;
     out->busy_seconds = st.busy_seconds;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_server_tuned_shape(sgpp_hip_server* h, int basis, int dim, int* found,
+                                            sgpp_hip_server_shape* out) {
+  return guarded([&] {
+    if (!h || !found) throw sgpp_hip::input_error("null argument");
+    if (basis != SGPP_HIP_BASIS_LINEAR && basis != SGPP_HIP_BASIS_MODLINEAR)
+      throw sgpp_hip::input_error("unknown basis");
+    sgpp_hip::TunedShape t;
+    *found = h->s->tuned_shape(
+                 basis == SGPP_HIP_BASIS_LINEAR ? sgpp_hip::Basis::Linear : sgpp_hip::Basis::ModLinear,
+                 dim, &t)
+                 ? 1
+                 : 0;
+    if (*found && out) {
+      out->chunk = t.chunk;
+      out->cost = t.cost;
+    }
     return SGPP_HIP_OK;
   });
 }

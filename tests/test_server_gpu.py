@@ -103,6 +103,40 @@ def test_alone_and_in_a_mix_are_identical(gpu):
     srv.close()
 
 
+def test_autotune_shapes_and_cache(gpu, tmp_path):
+    """Tuned shapes per (basis, dimension): valid chunks and costs, results
+    unchanged within tolerance, written to the cache file and read back by a
+    new server without tuning again."""
+    capi = gpu
+    cache = str(tmp_path / "shapes.txt")
+    srv = small_server(capi, 1, autotune=True, autotune_cache=cache)
+    models = small_models(capi)
+    ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    shapes = {}
+    for b, lv, ix, a in models:
+        sh = srv.tuned_shape(b, lv.shape[1])
+        assert sh is not None
+        assert sh["chunk"] >= 64 and sh["chunk"] % 64 == 0 and sh["cost"] > 0
+        shapes[(b, lv.shape[1])] = sh
+    for k, q in enumerate([3, 700, 4000]):
+        for m, (b, lv, ix, a) in zip(ids, models):
+            X = workloads.uniform_points(90 + k, q, lv.shape[1]).astype(np.float32)
+            X = X.astype(np.float64)
+            rid, n = srv.submit(m, X)
+            st, r, _ = srv.poll(rid, n)
+            assert st == capi.REQ_DONE
+            assert rel(r, oracle.mult(b, X, lv, ix, a)) <= 1e-5
+    srv.close()
+    with open(cache) as f:
+        assert f.read().count("sgpp_hip_server_shape") == len(models)
+    srv2 = small_server(capi, 1, autotune=True, autotune_cache=cache)
+    for (b, d), sh in shapes.items():
+        got = srv2.tuned_shape(b, d)
+        assert got["chunk"] == sh["chunk"]
+        assert abs(got["cost"] - sh["cost"]) <= 1e-6 * sh["cost"]
+    srv2.close()
+
+
 def test_failures_are_per_request(gpu):
     capi = gpu
     srv = small_server(capi, 1, max_inflight=1, max_requests_per_batch=1,
