@@ -483,7 +483,8 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         traffic = tj.get("bytes_per_launch")
-        executed = {k: tj.get(k) for k in ("valu_instr_per_pair_lane",
+        executed = {k: tj.get(k) for k in ("valu_instr_per_pair_lane", "valu_instr_per_launch",
+                                           "salu_instr_per_launch",
                                            "cycles_per_valu_instr_per_simd", "shader_clock_ghz",
                                            "algorithmic_bytes_per_launch")}
 
@@ -516,9 +517,13 @@ def main():
                      "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": traffic,
                      "flops_per_launch": flops_per_launch,
-                     "flop_convention": "6*d per pair (LearnerVectorizedPerformanceCalculator); "
-                                        "the ModLinear dimension skip executes fewer instructions "
-                                        "than 2d+1 per pair, so achieved can exceed the peak",
+                     "flop_convention": "6*d per pair for all M*N pairs "
+                                        "(LearnerVectorizedPerformanceCalculator); the kernels "
+                                        "evaluate only the pairs whose term can be nonzero "
+                                        "(support culling) and skip constant ModLinear factors, "
+                                        "so achieved under this convention exceeds the VALU peak; "
+                                        "the executed VALU rate is in 'executed' (rocprofv3 "
+                                        "counters)",
                      "executed": executed},
         "setup_s": setup_s,
     }
