@@ -71,14 +71,14 @@ constexpr int64_t kPartialBudget = int64_t(512) << 20;
 constexpr size_t kDataSlack = 1024;
 
 // Points per lane (K) and grid points evaluated together (J). Values from
-// variant sweeps on the d=10 ModLinear and d=8 Linear workloads.
+// variant sweeps on the d=10 ModLinear, d=8 Linear and d=20 Linear workloads.
 template <typename T>
 constexpr int mult_k(int D) {
   return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
 }
 template <typename T>
 constexpr int mult_j(int D) {
-  return 2;
+  return sizeof(T) == 4 && D > 10 ? 4 : 2;
 }
 template <typename T>
 constexpr int multT_k(int D) {
@@ -86,7 +86,7 @@ constexpr int multT_k(int D) {
 }
 template <typename T>
 constexpr int multT_j(int D) {
-  return 2;
+  return sizeof(T) == 4 && D > 10 ? 4 : 2;
 }
 // grid points per multTranspose block (FP64 accumulators in LDS, one row per wave)
 constexpr int kChunkT = 1024;
@@ -314,16 +314,36 @@ __global__ void prep_y_kernel(const double* __restrict__ y, const int32_t* __res
 }
 
 // out[perm ? perm[i] : i] = (scale ? scale[i] : 1) * sum_{p < nparts} part[p*ld + i],
-// summed in a fixed order
+// summed in the fixed order p = 0, 1, ... Each thread owns the pair i0, i0 + 1
+// (16-byte loads; ld is even and the rows are padded, so i0 + 1 < ld) and
+// issues eight rows' loads before their adds.
 __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld, int nparts,
                                     const double* __restrict__ scale,
                                     const int32_t* __restrict__ perm, double* __restrict__ out,
                                     int64_t n) {
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int p = 0; p < nparts; ++p) s += part[p * ld + i];
-  out[perm ? perm[i] : i] = scale ? s * scale[i] : s;
+  const int64_t i0 = 2 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
+  if (i0 >= n) return;
+  const double2* col = reinterpret_cast<const double2*>(part + i0);
+  const int64_t ld2 = ld / 2;
+  double s0 = 0.0, s1 = 0.0;
+  int p = 0;
+  for (; p + 8 <= nparts; p += 8) {
+    double2 v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = col[(p + u) * ld2];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      s0 += v[u].x;
+      s1 += v[u].y;
+    }
+  }
+  for (; p < nparts; ++p) {
+    const double2 v = col[p * ld2];
+    s0 += v.x;
+    s1 += v.y;
+  }
+  out[perm ? perm[i0] : i0] = scale ? s0 * scale[i0] : s0;
+  if (i0 + 1 < n) out[perm ? perm[i0 + 1] : i0 + 1] = scale ? s1 * scale[i0 + 1] : s1;
 }
 
 // ------------------------------------------------------------ dispatch
@@ -694,7 +714,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
   // results back to the caller's row order
   clear_launch_error();
-  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M_, 256)), dim3(256), 0, s, P,
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(M_, 512)), dim3(256), 0, s, P,
                      static_cast<int64_t>(Mpad_), G, static_cast<const double*>(nullptr),
                      static_cast<const int32_t*>(dperm_), d_result, static_cast<int64_t>(M_));
   check_hip(hipGetLastError(), "reduce (mult)");
@@ -753,7 +773,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
   if (tev) check_hip(hipEventRecord(tev[1], s), "hipEventRecord");
   // results back to the caller's grid order, times the per-point scale
   clear_launch_error();
-  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N_, 256)), dim3(256), 0, s, P,
+  hipLaunchKernelGGL(reduce_parts_kernel, dim3(ceil_div(N_, 512)), dim3(256), 0, s, P,
                      static_cast<int64_t>(Npad_), S, scale_, static_cast<const int32_t*>(perm_),
                      d_result, static_cast<int64_t>(N_));
   check_hip(hipGetLastError(), "reduce (multTranspose)");
