@@ -218,11 +218,12 @@ struct alignas(2 * sizeof(T)) LIPair {
 
 // Products p[q][k] = phi_{jj[q]}(x_k) of the J taken points of a tile. The
 // points' (L, I) pairs are read in batches of DB dimensions (wave-uniform
-// addresses: wide scalar loads). A dimension in which every taken point has
-// (L, I) = (0, 0) (a ModLinear level-1 factor, exactly 1) is skipped. The
+// addresses: wide scalar loads). With kSkip, a dimension in which every taken
+// point has (L, I) = (0, 0) (a ModLinear level-1 factor, exactly 1) is
+// skipped (Linear grids have no such dimension). The
 // batch's skip mask is computed from all its loaded pairs before any test, so
 // all loads of the batch are issued together, ahead of the branches.
-template <int D, int J, int K, typename T>
+template <int D, int J, int K, typename T, bool kSkip>
 __device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int (&jj)[J],
                                            const T (&x)[K][D], T (&p)[J][K]) {
   // dimensions per batch: the batch's pairs take at most 48 scalar registers
@@ -241,10 +242,10 @@ __device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int
 #pragma unroll
       for (int dd = 0; dd < DB; ++dd)
         if (d0 + dd < D) li[q][dd] = rows[jj[q] * D + d0 + dd];
-    uint32_t live = 0u;
+    uint32_t live = kSkip ? 0u : ~0u;
 #pragma unroll
     for (int dd = 0; dd < DB; ++dd) {
-      if (d0 + dd >= D) continue;
+      if (!kSkip || d0 + dd >= D) continue;
       auto any = bits_of(li[0][dd].L) | bits_of(li[0][dd].I);
 #pragma unroll
       for (int q = 1; q < J; ++q) any |= bits_of(li[q][dd].L) | bits_of(li[q][dd].I);

@@ -108,7 +108,7 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * 
 // X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] tile boxes,
 // A: [Npad] alpha*scale (zero padded),
 // part: [G][ldx]. Block: kBlock*K data points x grid chunk [y*chunk, +chunk).
-template <int D, int K, int J, typename T>
+template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
                                                       const float* __restrict__ TB,
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, i
       T a[J];
       take_hits<J, true, T>(bal, A, j0, jj, used, a);
       T p[J][K];
-      eval_group<D, J, K, T>(tile, jj, x, p);
+      eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
 #pragma unroll
       for (int q = 0; q < J; ++q)
 #pragma unroll
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // rows are added in a fixed order. Block: grid chunk
 // [blockIdx.x*kChunkT, +kChunkT) x data segment [blockIdx.y*seg, +seg), the
 // segment split into four contiguous wave shares. part: [S][ldp].
-template <int D, int K, int J, typename T>
+template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
         T unused_w[J];
         take_hits<J, false, T>(bal, static_cast<const T*>(nullptr), j0, jj, used, unused_w);
         T p[J][K];
-        eval_group<D, J, K, T>(tile, jj, x, p);
+        eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
 #pragma unroll
         for (int q = 0; q < J; ++q) {
           if (!used[q]) continue;
@@ -347,26 +347,27 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename T, int D>
+// kSkip: test for constant dimensions (ModLinear); Linear grids have none
+template <typename T, int D, bool kSkip>
 void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const T* A,
                    double* part, int chunk, int jend, int G, int64_t Mpad,
                    hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
-  hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, LI,
-                     TB, A, part, chunk, jend);
+  hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T, kSkip>), grid, dim3(kBlock), 0, s, X,
+                     ldx, LI, TB, A, part, chunk, jend);
 }
 
-template <typename T, int D>
+template <typename T, int D, bool kSkip>
 void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const float* TB,
                     double* part, int64_t ldp, int jend, int64_t seg,
                     int64_t mend, int S, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
   dim3 grid(static_cast<unsigned>((jend + kChunkT - 1) / kChunkT), static_cast<unsigned>(S));
   clear_launch_error();
-  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T>), grid, dim3(kBlock), 0, s, X, ldx, Y,
-                     LI, TB, part, ldp, jend, seg, mend);
+  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip>), grid, dim3(kBlock), 0, s, X,
+                     ldx, Y, LI, TB, part, ldp, jend, seg, mend);
 }
 
 template <typename T>
@@ -378,14 +379,17 @@ using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, do
 
 template <typename T, int... Ds>
 struct Table {
-  static MultFn<T> mult(int D) {
+  static MultFn<T> mult(int D, bool skip) {
     MultFn<T> f = nullptr;
-    ((D == Ds ? (f = &launch_mult_D<T, Ds>, 0) : 0), ...);
+    ((D == Ds ? (f = skip ? &launch_mult_D<T, Ds, true> : &launch_mult_D<T, Ds, false>, 0) : 0),
+     ...);
     return f;
   }
-  static MultTFn<T> multT(int D) {
+  static MultTFn<T> multT(int D, bool skip) {
     MultTFn<T> f = nullptr;
-    ((D == Ds ? (f = &launch_multT_D<T, Ds>, 0) : 0), ...);
+    ((D == Ds ? (f = skip ? &launch_multT_D<T, Ds, true> : &launch_multT_D<T, Ds, false>, 0)
+              : 0),
+     ...);
     return f;
   }
 };
@@ -700,7 +704,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
   double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
-    MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_));
+    MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_), basis_ == Basis::ModLinear);
     f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A_),
       P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
   } else {
@@ -758,7 +762,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
   double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (templ) {
-    MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_));
+    MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_), basis_ == Basis::ModLinear);
     f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
       P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
       s);

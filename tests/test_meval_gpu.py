@@ -206,6 +206,42 @@ def test_edge_cases(gpu):
         op.prepare(np.zeros((3, 2)), np.ones((3, 2)))  # level 0 is not a no-boundary point
 
 
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_culling_is_exact(gpu, basis, prec):
+    """Support culling drops only exact zeros: data on support boundaries
+    (dyadic coordinates, including 0 and 1), a box of zero width (one point
+    repeated) and a tight cluster beside uniform points, each checked against
+    the oracle, and a point outside every support reading exactly 0."""
+    capi = gpu
+    tol = 1e-13 if prec == 0 else 1e-6
+    rng = np.random.default_rng(7)
+    d = 3
+    g = capi.Grid(basis, d).regular(5)
+    lv, ix = g.level_index()
+    alpha = workloads.normal(workloads.stream(41, 1), g.size)
+    dyadic = rng.integers(0, 65, size=(5000, d)) / 64.0
+    same = np.tile(np.array([[0.3125, 0.6875, 0.5]]), (3000, 1))
+    cluster = np.vstack([0.8 + 0.001 * rng.random((2500, d)), rng.random((2500, d))])
+    if prec == 1:
+        cluster = cluster.astype(np.float32).astype(np.float64)
+    for X in (dyadic, same, cluster):
+        y = workloads.normal(workloads.stream(42, X.shape[0]), X.shape[0])
+        op = capi.MultiEval(basis, X, precision=prec).prepare(lv, ix)
+        r, a = op.mult(alpha), op.mult_transpose(y)
+        r0 = oracle.mult(basis, X, lv, ix, alpha)
+        a0 = oracle.mult_transpose(basis, X, lv, ix, y)
+        assert rel(r, r0) <= tol and rel(a, a0) <= tol
+        assert np.max(np.abs(r - r0)) <= tol * np.max(np.abs(r0))
+    if basis == 0:
+        # every Linear basis function vanishes at the corner x = (0, 0, 0), so
+        # every multTranspose entry is an exact zero
+        X = np.zeros((64, d))
+        op = capi.MultiEval(basis, X, precision=prec).prepare(lv, ix)
+        a = op.mult_transpose(np.ones(64))
+        assert np.all(a == 0.0)
+
+
 def test_launch_shapes_agree_and_are_deterministic(gpu):
     capi = gpu
     g, X, alpha, y = make_case(capi, 1, 6, 4, 20000, seed=31)
