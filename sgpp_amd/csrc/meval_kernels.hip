@@ -232,6 +232,9 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
       if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
       const T* tile = LI + t * D * kTG * 2;
       unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
+      // a grid point is taken at most once per (group, tile): its sum goes
+      // to lane jj of tsum, and the tile is added to the wave's row once
+      T tsum = T(0);
       while (bal) {
         int jj[J];
         bool used[J];
@@ -245,12 +248,13 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
           T c = T(0);
 #pragma unroll
           for (int k = 0; k < K; ++k) c = fmaT(y[k], p[q][k], c);
-          c = sum_to_lane63(c);
-          // one lane adds to the wave's own accumulator row (LDS atomics of
-          // one wave complete in issue order: deterministic)
-          if (lane == 63) atomicAdd(&acc[j0 - jb + jj[q]], static_cast<double>(c));
+          const T sq = lane63(sum_to_lane63(c));
+          tsum = lane == jj[q] ? sq : tsum;
         }
       }
+      // one vector add per tile into the wave's own accumulator row (in
+      // issue order: deterministic; untaken lanes add +0)
+      atomicAdd(&acc[j0 - jb + lane], static_cast<double>(tsum));
     }
   }
   __syncthreads();
