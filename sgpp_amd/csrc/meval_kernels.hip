@@ -190,11 +190,11 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // Same orientation as mult: lanes own data points (K per lane, coordinates
 // and y in registers), grid tiles are culled against the wave's data box and
 // the surviving points are evaluated J at a time. A point's contribution
-// sum_k y_k phi(x_k) is summed over the wave's lanes (butterfly) and added,
-// in FP64, to the wave's own row of LDS accumulators; at the end the four
-// rows are added in a fixed order. Block: grid chunk
-// [blockIdx.x*kChunkT, +kChunkT) x data segment [blockIdx.y*seg, +seg), the
-// segment split into four contiguous wave shares. part: [S][ldp].
+// sum_k y_k phi(x_k) is summed over the wave's lanes (DPP) and added, in
+// FP64, to the wave's own row of LDS accumulators; at the end the four rows
+// are added in a fixed order. Block: data segment [blockIdx.x*seg, +seg) x
+// grid chunk [blockIdx.y*kChunkT, +kChunkT), the segment split into four
+// contiguous wave shares. part: [S][ldp].
 template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
@@ -205,11 +205,11 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
   __shared__ double accs[kBlock / 64][kChunkT];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  const int jb = blockIdx.x * kChunkT;
+  const int jb = blockIdx.y * kChunkT;
   const int je = min(jb + kChunkT, jend_all);
   for (int i = threadIdx.x; i < (kBlock / 64) * kChunkT; i += kBlock) (&accs[0][0])[i] = 0.0;
   __syncthreads();
-  const int64_t mb = static_cast<int64_t>(blockIdx.y) * seg;
+  const int64_t mb = static_cast<int64_t>(blockIdx.x) * seg;
   const int64_t me = min(mb + seg, mend);
   // per-wave share, a multiple of 64*K; points past the share weigh 0
   const int64_t share = ((me - mb) / (kBlock / 64) + 64 * K - 1) / (64 * K) * (64 * K);
@@ -258,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, 
     }
   }
   __syncthreads();
-  double* out = part + static_cast<int64_t>(blockIdx.y) * ldp;
+  double* out = part + static_cast<int64_t>(blockIdx.x) * ldp;
   for (int i = threadIdx.x; i < je - jb; i += kBlock) {
     double s = accs[0][i];
 #pragma unroll
@@ -368,7 +368,9 @@ void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const floa
                     double* part, int64_t ldp, int jend, int64_t seg,
                     int64_t mend, int S, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
-  dim3 grid(static_cast<unsigned>((jend + kChunkT - 1) / kChunkT), static_cast<unsigned>(S));
+  // data segments vary fastest: consecutive blocks share one grid chunk
+  // (its rows stay hot in every XCD's caches) and read different data
+  dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>((jend + kChunkT - 1) / kChunkT));
   clear_launch_error();
   hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip>), grid, dim3(kBlock), 0, s, X,
                      ldx, Y, LI, TB, part, ldp, jend, seg, mend);
