@@ -276,3 +276,60 @@ def test_device_pointers_and_adjoint_identity(gpu):
         scale = float(torch.linalg.norm(r) * torch.linalg.norm(y))
         assert abs(lhs - rhs) <= tol * scale
         assert op.kernel_ms() > 0
+
+
+def test_full_size_properties(gpu):
+    """The full ModLinear d=10 workload (N = 100,015 adaptive grid points,
+    M = 1,000,000 Gaussian-mixture points), checked through properties that
+    hold at any size: the adjoint identity <B a, y> = <a, B^T y>, linearity
+    B(a1 + 2 a2) = B a1 + 2 B a2 and the same for B^T, FP32 against FP64, and
+    bit-identical repeats. Sampled rows and columns are also checked against
+    the oracle."""
+    import torch
+    capi = gpu
+    c = workloads.config_modlinear_adaptive()
+    lv, ix = c["grid"].level_index()
+    X = c["X"]
+    M, N = X.shape[0], lv.shape[0]
+    assert (M, N) == (1000000, 100015)
+    dev = torch.device("cuda:0")
+    a1 = torch.tensor(c["alpha"], device=dev)
+    a2 = torch.tensor(workloads.normal(workloads.stream(77, 1), N), device=dev)
+    y1 = torch.tensor(c["y"], device=dev)
+    y2 = torch.tensor(workloads.normal(workloads.stream(77, 2), M), device=dev)
+    op = capi.MultiEval(1, X, precision=1, keep_fp64=True).prepare(lv, ix)
+
+    def mult(a, prec):
+        r = torch.empty(M, dtype=torch.float64, device=dev)
+        op.mult_device(a.data_ptr(), r.data_ptr(), prec)
+        op.synchronize()
+        return r
+
+    def mult_t(y, prec):
+        r = torch.empty(N, dtype=torch.float64, device=dev)
+        op.mult_transpose_device(y.data_ptr(), r.data_ptr(), prec)
+        op.synchronize()
+        return r
+
+    def rel_t(a, b):
+        return float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+
+    r64, t64 = mult(a1, 0), mult_t(y1, 0)
+    # adjoint identity
+    lhs, rhs = float(torch.dot(r64, y1)), float(torch.dot(a1, t64))
+    assert abs(lhs - rhs) <= 1e-12 * float(torch.linalg.norm(r64) * torch.linalg.norm(y1))
+    # linearity
+    assert rel_t(mult(a1 + 2 * a2, 0), r64 + 2 * mult(a2, 0)) <= 1e-12
+    assert rel_t(mult_t(y1 + 2 * y2, 0), t64 + 2 * mult_t(y2, 0)) <= 1e-12
+    # repeats are bit-identical
+    assert torch.equal(mult(a1, 0), r64) and torch.equal(mult_t(y1, 0), t64)
+    assert torch.equal(mult(a1, 1), mult(a1, 1)) and torch.equal(mult_t(y1, 1), mult_t(y1, 1))
+    # FP32 products with FP64 accumulation against FP64
+    assert rel_t(mult(a1, 1), r64) <= 1e-5 and rel_t(mult_t(y1, 1), t64) <= 1e-5
+    # sampled rows / columns against the oracle
+    rows = np.arange(0, M, M // 97)[:97]
+    cols = np.arange(0, N, N // 13)[:13]
+    r0 = oracle.mult(1, X[rows], lv, ix, c["alpha"])
+    t0 = oracle.mult_transpose(1, X, lv[cols], ix[cols], c["y"])
+    assert rel(r64.cpu().numpy()[rows], r0) <= 1e-12
+    assert rel(t64.cpu().numpy()[cols], t0) <= 1e-12
