@@ -23,11 +23,12 @@
 // full evaluation.
 //   mult (r = B alpha): blocks = data tiles (x) x G grid chunks (y); the G
 //     FP64 partial rows are summed in a fixed order by reduce_parts.
-//   multTranspose (a = B^T y): blocks = grid chunks of kChunkT points (x) x
-//     S data segments (y); per taken grid point the wave's sum over its lanes
-//     (DPP) is added to the wave's row of FP64 LDS accumulators; the four rows
-//     and the S partial rows are summed in a fixed order. No global atomics:
-//     results are deterministic.
+//   multTranspose (a = B^T y): blocks = data segments (x) x grid chunks of
+//     kChunkT points (y); per taken grid point the wave's sum over its lanes
+//     (DPP) goes to a per-tile register, and each tile is added to the
+//     wave's row of FP64 LDS accumulators; the four rows and the S partial
+//     rows are summed in a fixed order. No global atomics: results are
+//     deterministic.
 // Precision: FP64 throughout, or FP32 basis products and per-tile sums with
 //   the per-tile partials accumulated in FP64 registers and FP64 partials.
 //
@@ -53,8 +54,8 @@ namespace sgpp_hip {
 namespace {
 
 constexpr int kBlock = 256;  // 4 waves of 64
-constexpr int kTG = 64;      // grid points per LDS tile (mult) and per mask tile
-constexpr int kTM = 128;     // data points per LDS tile (multTranspose)
+constexpr int kTG = 64;      // grid points per tile (culling unit, one point per lane)
+constexpr int kTM = 128;     // data points per step of the generic multTranspose
 constexpr int kMaxTemplD = 20;
 // Default launch sizes. Culled blocks cost very different amounts, so many
 // short blocks balance better: mult ~128 workgroups per CU and grid chunks of
@@ -105,9 +106,10 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * 
 //   3. evaluates only points that may be nonzero in the box, J at a time,
 //      over the union of their non-constant dimensions.
 // Every dropped term is exactly zero, so the sums equal a full evaluation.
-// X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] tile boxes,
-// A: [Npad] alpha*scale (zero padded),
-// part: [G][ldx]. Block: kBlock*K data points x grid chunk [y*chunk, +chunk).
+// X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] FP32 tile
+// boxes, A: [Npad] alpha*scale (zero padded), part: [G][ldx]. Block: kBlock*K
+// data points (each wave 64*K consecutive ones) x grid chunk [y*chunk,
+// +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
 template <int D, int K, int J, typename T, bool kSkip>
 __global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
@@ -190,9 +192,10 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // Same orientation as mult: lanes own data points (K per lane, coordinates
 // and y in registers), grid tiles are culled against the wave's data box and
 // the surviving points are evaluated J at a time. A point's contribution
-// sum_k y_k phi(x_k) is summed over the wave's lanes (DPP) and added, in
-// FP64, to the wave's own row of LDS accumulators; at the end the four rows
-// are added in a fixed order. Block: data segment [blockIdx.x*seg, +seg) x
+// sum_k y_k phi(x_k) is summed over the wave's lanes (DPP) into lane jj of a
+// per-tile register (a point is taken at most once per group and tile); the
+// tile is then added, in FP64, to the wave's own row of LDS accumulators. At
+// the end the four rows are added in a fixed order. Block: data segment [blockIdx.x*seg, +seg) x
 // grid chunk [blockIdx.y*kChunkT, +kChunkT), the segment split into four
 // contiguous wave shares. part: [S][ldp].
 template <int D, int K, int J, typename T, bool kSkip>
