@@ -65,17 +65,55 @@ constexpr int serve_j(int D) {
                                                           : 8;
 }
 
+// LDS staging buffer of one 64-point grid tile: weights A (kTG values),
+// per-4-point masks (kTG/4 words), then the tile-major (L, I) pairs
+// (D*kTG*2 values); every part starts on a 16-byte boundary.
+template <typename T>
+constexpr int stage_a_off() { return 0; }
+template <typename T>
+constexpr int stage_m_off() { return kTG * static_cast<int>(sizeof(T)); }
+template <typename T>
+constexpr int stage_li_off() { return stage_m_off<T>() + kTG; }
+template <typename T>
+constexpr int stage_bytes(int D) { return stage_li_off<T>() + kTG * D * 2 * static_cast<int>(sizeof(T)); }
+
+// Block-wide copy of n 16-byte chunks from global memory straight into LDS
+// (global_load_lds_dwordx4: no registers; completion via vmcnt). Each wave
+// copies 64 consecutive chunks per instruction; dst is wave-uniform.
+__device__ __forceinline__ void copy_to_lds(const uint4* __restrict__ src, uint4* dst, int n,
+                                            int wave, int lane) {
+  for (int c0 = wave * 64; c0 < n; c0 += kBlock)
+    if (c0 + lane < n) __builtin_amdgcn_global_load_lds(src + c0 + lane, dst + c0, 16, 0, 0);
+}
+
+// Starts the copy of grid tile j0 of model md into the staging buffer buf.
+template <int D, typename T>
+__device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned char* buf) {
+  const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
+  const unsigned char* A = static_cast<const unsigned char*>(md.A);
+  const unsigned char* M = reinterpret_cast<const unsigned char*>(md.qmask);
+  const unsigned char* LI = static_cast<const unsigned char*>(md.LI);
+  copy_to_lds(reinterpret_cast<const uint4*>(A + static_cast<long long>(j0) * sizeof(T)),
+              reinterpret_cast<uint4*>(buf + stage_a_off<T>()), kTG * sizeof(T) / 16, wave, lane);
+  copy_to_lds(reinterpret_cast<const uint4*>(M + j0), reinterpret_cast<uint4*>(buf + stage_m_off<T>()),
+              kTG / 16, wave, lane);
+  copy_to_lds(reinterpret_cast<const uint4*>(LI + static_cast<long long>(j0) * D * 2 * sizeof(T)),
+              reinterpret_cast<uint4*>(buf + stage_li_off<T>()), kTG * D * 2 * sizeof(T) / 16,
+              wave, lane);
+}
+
 // one work item: a tile of kBlock*K queries of one request against one grid
-// chunk. The grid tile is walked in sub-tiles of J points; per (sub-tile,
-// dimension) one wave-uniform test of the sub-tile's mask decides whether the
-// dimension is evaluated (ModLinear level-1 factors are exactly 1; Linear
-// models carry full masks).
+// chunk. The chunk's 64-point grid tiles are staged in LDS, double-buffered:
+// the copy of tile t+1 runs while tile t is evaluated, one barrier per tile.
+// A tile is walked in sub-tiles of J points; per (sub-tile, dimension) one
+// wave-uniform test of the sub-tile's mask decides whether the dimension is
+// evaluated (ModLinear level-1 factors are exactly 1; Linear models carry
+// full masks).
 template <int D, int K, int J, typename T>
 __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md, const T* X,
-                                           double* part, T* sLI, T* sA, uint32_t* sM) {
+                                           double* part, unsigned char* sbuf, int bufbytes) {
   static_assert(kTG % J == 0 && J % 4 == 0, "sub-tile must divide the tile, 4 must divide it");
-  const T* LI = static_cast<const T*>(md.LI);
-  const T* A = static_cast<const T*>(md.A);
   const T* Xr = X + item.xoff;
   T x[K][D];
 #pragma unroll
@@ -86,13 +124,16 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
+  int cur = 0;
+  stage_tile<D, T>(md, item.j0, sbuf);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
   for (int j0 = item.j0; j0 < item.j1; j0 += kTG) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kTG * D * 2; e += kBlock)
-      sLI[e] = LI[static_cast<long long>(j0) * D * 2 + e];
-    if (threadIdx.x < kTG) sA[threadIdx.x] = A[j0 + threadIdx.x];
-    if (threadIdx.x < kTG / 4) sM[threadIdx.x] = md.qmask[j0 / 4 + threadIdx.x];
-    __syncthreads();
+    if (j0 + kTG < item.j1) stage_tile<D, T>(md, j0 + kTG, sbuf + (cur ^ 1) * bufbytes);
+    const unsigned char* buf = sbuf + cur * bufbytes;
+    const T* sA = reinterpret_cast<const T*>(buf + stage_a_off<T>());
+    const uint32_t* sM = reinterpret_cast<const uint32_t*>(buf + stage_m_off<T>());
+    const T* sLI = reinterpret_cast<const T*>(buf + stage_li_off<T>());
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
@@ -129,6 +170,10 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
+    // the next tile's copy has landed and every wave is done with this one
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    cur ^= 1;
   }
   double* out = part + item.poff;
 #pragma unroll
@@ -141,8 +186,8 @@ int serve_k_rt(Precision p, int D) {
 
 template <typename T, int D>
 __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md, const T* X,
-                                             double* part, T* sLI, T* sA, uint32_t* sM) {
-  serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sLI, sA, sM);
+                                             double* part, unsigned char* sbuf) {
+  serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sbuf, stage_bytes<T>(kMaxD));
 }
 
 // Persistent work-list kernel: blocks pull items from a counter until the
@@ -153,9 +198,8 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restri
                                                        const int* __restrict__ counts,
                                                        unsigned* counter, const T* __restrict__ X,
                                                        double* __restrict__ part) {
-  __shared__ T sLI[kTG * kMaxD * 2];
-  __shared__ T sA[kTG];
-  __shared__ uint32_t sM[kTG / 4];
+  // two staging buffers (double-buffered grid tiles)
+  __shared__ __align__(16) unsigned char sbuf[2 * stage_bytes<T>(kMaxD)];
   __shared__ unsigned s_item;
   const unsigned n = static_cast<unsigned>(counts[0]);
   for (;;) {
@@ -169,7 +213,7 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restri
     switch (item.D) {
 #define SGPP_CASE(DD) \
   case DD:            \
-    dispatch_one<T, DD>(item, md, X, part, sLI, sA, sM); \
+    dispatch_one<T, DD>(item, md, X, part, sbuf); \
     break;
       SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
       SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
