@@ -321,14 +321,21 @@ struct Server::Request {
   uint64_t id = 0;
   int model = 0;
   size_t q = 0;
-  size_t ldq = 0;
-  size_t qoff = 0;      // column of the first query in its model's block of the batch
+  size_t next = 0;      // first query not yet placed in a batch
+  size_t done = 0;      // queries whose results have arrived
   std::vector<char> X;  // [D][q] in the compute type
   std::vector<double> out;
   double t_submit = 0.0, t_done = 0.0, deadline = 0.0;
-  int status = REQ_QUEUED;
+  int status = REQ_QUEUED;  // QUEUED while queries remain unplaced
   std::string error;
-  // batch placement
+};
+
+// Queries [q0, q0 + nq) of a request, placed in one batch: columns
+// [qoff, qoff + nq) of its model's block at xoff (leading dimension ldq);
+// results at out_off of the batch output.
+struct Server::Piece {
+  std::shared_ptr<Request> r;
+  size_t q0 = 0, nq = 0, qoff = 0, ldq = 0;
   long long xoff = 0, out_off = 0;
 };
 
@@ -352,7 +359,7 @@ struct Server::Slot {
   // current batch
   bool busy = false;
   bool inject = false;
-  std::vector<std::shared_ptr<Request>> reqs;
+  std::vector<Piece> pieces;
   struct Block {
     long long xoff;
     int D;
@@ -627,6 +634,7 @@ uint64_t Server::submit(int model, size_t q, const double* queries, double deadl
   auto r = std::make_shared<Request>();
   r->model = model;
   r->q = q;
+  r->out.assign(q, 0.0);
   const size_t D = m->D;
   for (size_t i = 0; i < q * D; ++i)
     if (!(queries[i] >= 0.0 && queries[i] <= 1.0))
@@ -706,20 +714,31 @@ void Server::fail_request(const std::shared_ptr<Request>& r, int status, const c
   done_cv_.notify_all();
 }
 
-// caller holds mu_: move requests from the queue into slot s (FIFO)
+// caller holds mu_: the batch of slot s failed on the device; its requests
+// fail (a request split over several batches fails once, at its first failed
+// piece) and the slot is free again
+void Server::fail_pieces(Slot& s, const char* why) {
+  for (const Piece& pc : s.pieces)
+    if (pc.r->status == REQ_QUEUED || pc.r->status == REQ_RUNNING)
+      fail_request(pc.r, REQ_FAILED_DEVICE, why);
+  s.pieces.clear();
+}
+
 // caller holds mu_: move requests from the queue into slot s (FIFO). The
 // queries of all requests for one model form one block [D][ldq] of the batch
 // (in queue order), tiled and chunked together: a query's result depends only
 // on the model and its grid chunks, so packing changes no result bit, and a
 // request pays for at most one partially filled tile per model and batch.
-bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
+// When the rest of the front request does not fit, its longest prefix that
+// does is placed and the request stays at the front for the next batch.
+bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
   s.usedX = s.nItems = s.nRed = s.usedPart = s.usedOut = 0;
   s.blocks.clear();
   const double t = now_s();
   struct Group {
     const Model* m;
     size_t tile, chunk, nchunks, q;
-    std::vector<std::shared_ptr<Request>> reqs;
+    std::vector<size_t> pieces;  // indices into taken
   };
   std::vector<Group> groups;
   std::map<int, size_t> gidx;
@@ -735,11 +754,11 @@ bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
   size_t useX = 0, useItems = 0, usePart = 0, useRed = 0, useOut = 0;
   while (!queue_.empty() && taken.size() < static_cast<size_t>(cfg_.max_requests_per_batch)) {
     std::shared_ptr<Request> r = queue_.front();
-    if (r->status != REQ_QUEUED) {  // cancelled meanwhile
+    if (r->status != REQ_QUEUED) {  // cancelled or failed meanwhile
       queue_.pop_front();
       continue;
     }
-    if (r->deadline > 0 && t > r->deadline) {
+    if (r->next == 0 && r->deadline > 0 && t > r->deadline) {
       queue_.pop_front();
       fail_request(r, REQ_FAILED_TIMEOUT, "deadline passed before the request was scheduled");
       continue;
@@ -749,47 +768,70 @@ bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
                 0, {}};
     probe.nchunks = (m.Npad + probe.chunk - 1) / probe.chunk;
     size_t x1, i1, p1, r1;
-    need(probe, r->q, &x1, &i1, &p1, &r1);
-    if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX || r->q > s.capOut || r1 > s.capRed) {
+    need(probe, 1, &x1, &i1, &p1, &r1);
+    if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX || r1 > s.capRed) {
       queue_.pop_front();
-      fail_request(r, REQ_FAILED_BUDGET, "request exceeds the batch buffer capacity");
+      fail_request(r, REQ_FAILED_BUDGET, "one query tile of this model exceeds the batch buffers");
       continue;
     }
     auto gi = gidx.find(r->model);
     const Group& g = gi == gidx.end() ? probe : groups[gi->second];
     size_t x0 = 0, i0 = 0, p0 = 0, r0 = 0;
     if (gi != gidx.end()) need(g, g.q, &x0, &i0, &p0, &r0);
-    need(g, g.q + r->q, &x1, &i1, &p1, &r1);
-    if (useX + x1 - x0 > s.capX || useItems + i1 - i0 > s.capItems ||
-        usePart + p1 - p0 > s.capPart || useRed + r1 - r0 > s.capRed ||
-        useOut + r->q > s.capOut)
-      break;  // full: the rest waits for the next slot (FIFO)
+    // does a piece of n more queries fit? (monotone in n)
+    auto fits = [&](size_t n) {
+      need(g, g.q + n, &x1, &i1, &p1, &r1);
+      return useX + x1 - x0 <= s.capX && useItems + i1 - i0 <= s.capItems &&
+             usePart + p1 - p0 <= s.capPart && useRed + r1 - r0 <= s.capRed &&
+             useOut + n <= s.capOut;
+    };
+    const size_t rest = r->q - r->next;
+    size_t n = rest;
+    if (!fits(rest)) {
+      size_t lo = 0, hi = rest;  // fits(lo), !fits(hi)
+      while (hi - lo > 1) {
+        const size_t mid = lo + (hi - lo) / 2;
+        (fits(mid) ? lo : hi) = mid;
+      }
+      n = lo;
+      if (n == 0) break;  // full: the rest waits for the next slot (FIFO)
+    }
+    need(g, g.q + n, &x1, &i1, &p1, &r1);
     useX += x1 - x0;
     useItems += i1 - i0;
     usePart += p1 - p0;
     useRed += r1 - r0;
-    useOut += r->q;
-    queue_.pop_front();
-    r->status = REQ_RUNNING;
+    useOut += n;
     if (gi == gidx.end()) {
       gidx[r->model] = groups.size();
       groups.push_back(probe);
     }
     Group& gg = groups[gidx[r->model]];
-    r->qoff = gg.q;
-    gg.q += r->q;
-    gg.reqs.push_back(r);
-    taken.push_back(r);
+    Piece pc;
+    pc.r = r;
+    pc.q0 = r->next;
+    pc.nq = n;
+    pc.qoff = gg.q;
+    gg.q += n;
+    gg.pieces.push_back(taken.size());
+    taken.push_back(pc);
+    r->next += n;
+    if (r->next == r->q) {
+      queue_.pop_front();
+      r->status = REQ_RUNNING;
+    }
+    if (n < rest) break;  // the batch is full
   }
   for (Group& g : groups) {
     const size_t ntiles = (g.q + g.tile - 1) / g.tile;
     const size_t ldq = ntiles * g.tile;
     const long long xoff = static_cast<long long>(s.usedX);
     const long long out0 = static_cast<long long>(s.usedOut);
-    for (auto& r : g.reqs) {
-      r->xoff = xoff;
-      r->ldq = ldq;
-      r->out_off = out0 + static_cast<long long>(r->qoff);
+    for (size_t k : g.pieces) {
+      Piece& pc = taken[k];
+      pc.xoff = xoff;
+      pc.ldq = ldq;
+      pc.out_off = out0 + static_cast<long long>(pc.qoff);
     }
     add_items(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q, ldq, xoff, out0,
               s.usedPart, s.hItems + s.nItems, s.hRed + s.nRed);
@@ -806,11 +848,12 @@ bool Server::form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken) {
 void Server::launch_batch(Slot& s) {
   const size_t esz = cfg_.precision == Precision::FP64 ? 8 : 4;
   // each request's rows into its model's block, padding columns zero
-  for (auto& r : s.reqs) {
-    const size_t D = r->X.size() / (r->q * esz);
+  for (const Piece& pc : s.pieces) {
+    const Request& r = *pc.r;
+    const size_t D = r.X.size() / (r.q * esz);
     for (size_t d = 0; d < D; ++d)
-      std::memcpy(s.hX + (r->xoff + d * r->ldq + r->qoff) * esz, r->X.data() + d * r->q * esz,
-                  r->q * esz);
+      std::memcpy(s.hX + (pc.xoff + d * pc.ldq + pc.qoff) * esz,
+                  r.X.data() + (d * r.q + pc.q0) * esz, pc.nq * esz);
   }
   for (const Slot::Block& b : s.blocks)
     for (int d = 0; d < b.D; ++d)
@@ -858,24 +901,28 @@ void Server::finish_batch(Slot& s) {
   if (hipEventElapsedTime(&ms, s.ev_start, s.ev_done) == hipSuccess)
     stats_.busy_seconds += ms * 1e-3;
   const double t = now_s();
-  for (auto& r : s.reqs) {
+  for (const Piece& pc : s.pieces) {
+    Request& r = *pc.r;
+    if (r.status != REQ_QUEUED && r.status != REQ_RUNNING) continue;  // failed or cancelled
     if (s.inject) {
-      fail_request(r, REQ_FAILED_DEVICE, "injected kernel error");
+      fail_request(pc.r, REQ_FAILED_DEVICE, "injected kernel error");
       continue;
     }
-    r->out.assign(s.hOut + r->out_off, s.hOut + r->out_off + r->q);
-    r->X.clear();
-    r->X.shrink_to_fit();
-    r->status = REQ_DONE;
-    r->t_done = t;
-    auto m = models_.find(r->model);
+    std::memcpy(r.out.data() + pc.q0, s.hOut + pc.out_off, pc.nq * sizeof(double));
+    r.done += pc.nq;
+    if (r.done < r.q) continue;
+    r.X.clear();
+    r.X.shrink_to_fit();
+    r.status = REQ_DONE;
+    r.t_done = t;
+    auto m = models_.find(r.model);
     if (m != models_.end()) m->second->active--;
     stats_.requests_done++;
-    stats_.queries_done += r->q;
+    stats_.queries_done += r.q;
   }
   stats_.batches++;
   stats_.kernel_launches += cfg_.use_graph ? 1 : 3;
-  s.reqs.clear();
+  s.pieces.clear();
   s.busy = false;
   s.inject = false;
   running_--;
@@ -1079,8 +1126,7 @@ void Server::loop() {
       } else if (q == hipErrorNotReady) {
         any_busy = true;
       } else {
-        for (auto& r : s.reqs) fail_request(r, REQ_FAILED_DEVICE, hipGetErrorString(q));
-        s.reqs.clear();
+        fail_pieces(s, hipGetErrorString(q));
         s.busy = false;
         running_--;
       }
@@ -1088,9 +1134,9 @@ void Server::loop() {
     for (auto& sp : slots_) {
       Slot& s = *sp;
       if (s.busy || queue_.empty()) continue;
-      std::vector<std::shared_ptr<Request>> taken;
+      std::vector<Piece> taken;
       if (!form_batch(s, taken)) continue;
-      s.reqs = std::move(taken);
+      s.pieces = std::move(taken);
       s.busy = true;
       s.inject = inject_error_.exchange(false);
       running_++;
@@ -1105,8 +1151,7 @@ void Server::loop() {
       }
       lk.lock();
       if (!ok) {
-        for (auto& r : s.reqs) fail_request(r, REQ_FAILED_DEVICE, err.c_str());
-        s.reqs.clear();
+        fail_pieces(s, err.c_str());
         s.busy = false;
         running_--;
       } else {

@@ -24,7 +24,10 @@
 //   * partials of a query are summed over its model's grid chunks in chunk
 //     order, and a query's result depends only on its model and that
 //     chunking, so a result is bit-identical whether the request runs alone
-//     or in any mix.
+//     or in any mix;
+//   * a request larger than a batch's buffers is split by queries into
+//     pieces over successive batches (results unchanged, by the above); only
+//     a model whose single query tile exceeds them fails "over budget".
 // Per-request failures (over budget, deadline passed, cancelled, an injected
 // kernel-error flag) are reported on the request and never stop the loop.
 #pragma once
@@ -121,6 +124,7 @@ class Server {
  private:
   struct Model;
   struct Request;
+  struct Piece;
   struct Slot;
   size_t chunk_for(const Model& m) const;
   void tune(Model& m);
@@ -128,10 +132,11 @@ class Server {
   void save_cache();
   static int tune_key(Basis basis, int D);
   void loop();
-  bool form_batch(Slot& s, std::vector<std::shared_ptr<Request>>& taken);
+  bool form_batch(Slot& s, std::vector<Piece>& taken);
   void launch_batch(Slot& s);
   void finish_batch(Slot& s);
   void fail_request(const std::shared_ptr<Request>& r, int status, const char* why);
+  void fail_pieces(Slot& s, const char* why);
 
   ServerConfig cfg_;
   int num_cus_ = 256;

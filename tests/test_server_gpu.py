@@ -103,6 +103,45 @@ def test_alone_and_in_a_mix_are_identical(gpu):
     srv.close()
 
 
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (0, 1e-12)])
+def test_large_requests_are_split(gpu, prec, tol):
+    """Requests larger than a batch's buffers (queries, work items, partials)
+    run as pieces over several batches, interleaved with other traffic; every
+    result is bit-identical to the same request on a server whose batches
+    hold it whole, and within tolerance of the oracle."""
+    capi = gpu
+    models = small_models(capi)
+    whole = small_server(capi, prec)
+    split = small_server(capi, prec, max_batch_queries=1 << 12, max_items=64,
+                         max_partial_doubles=1 << 16)
+    ids_w = [whole.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    ids_s = [split.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    reqs = []
+    for k, q in enumerate([20000, 9000]):
+        for j, (b, lv, ix, a) in enumerate(models):
+            X = workloads.uniform_points(110 + 4 * k + j, q, lv.shape[1])
+            if prec == 1:
+                X = X.astype(np.float32).astype(np.float64)
+            small = split.submit(ids_s[j], X[:q // 50])  # traffic between the pieces
+            reqs.append((split.submit(ids_s[j], X), whole.submit(ids_w[j], X), small, X, j))
+    n_split = 0
+    for (rs, n), (rw, _), small, X, j in reqs:
+        st, got, _ = split.poll(rs, n)
+        assert st == capi.REQ_DONE, capi.REQ_NAMES[st]
+        st, ref, _ = whole.poll(rw, n)
+        assert st == capi.REQ_DONE
+        assert np.array_equal(got, ref), "a split request differs from the whole one"
+        b, lv, ix, a = models[j]
+        assert rel(got, oracle.mult(b, X, lv, ix, a)) <= tol
+        assert split.poll(*small)[0] == capi.REQ_DONE
+        n_split += n > (1 << 12)
+    s = split.stats()
+    assert s["requests_failed"] == 0 and s["requests_done"] == 2 * len(reqs)
+    assert s["batches"] >= n_split * 3  # every large request took several batches
+    split.close()
+    whole.close()
+
+
 def test_autotune_shapes_and_cache(gpu, tmp_path):
     """Tuned shapes per (basis, dimension): valid chunks and costs, results
     unchanged within tolerance, written to the cache file and read back by a
@@ -154,9 +193,16 @@ def test_failures_are_per_request(gpu):
                          np.ones(lv.shape[0] * 64))
     assert e.value.status_name == "BUDGET"
     tiny.close()
-    # a request larger than a batch can hold fails alone
+    # a request larger than a batch is split and completes
     rid, n = srv.submit(ids[0], workloads.uniform_points(90, 70000, d0))
-    assert srv.poll(rid, n)[0] == capi.REQ_FAILED_BUDGET
+    assert srv.poll(rid, n)[0] == capi.REQ_DONE
+    # a model whose single query tile needs more work items than a batch
+    # holds (64-point chunks, one item per batch) fails alone, over budget
+    one = small_server(capi, 1, max_items=1, item_work=1.0)
+    m0 = one.admit_model(*models[0])
+    rid, n = one.submit(m0, workloads.uniform_points(90, 10, d0))
+    assert one.poll(rid, n)[0] == capi.REQ_FAILED_BUDGET
+    one.close()
     # a deadline that has passed before scheduling
     heavy = srv.submit(ids[2], workloads.uniform_points(91, 60000, models[2][1].shape[1]))
     late = srv.submit(ids[0], workloads.uniform_points(92, 10, d0), deadline_s=1e-7)
