@@ -31,26 +31,51 @@ struct ModelDesc {
   int Npad;               // multiple of kTG
 };
 
-struct Item {
-  int model;
+// The queries of one model in a batch ("span"): stored [D][ldq] at xoff and
+// cut into ldq / tile query tiles. Its work items are the (query tile, grid
+// chunk) pairs, tile-major, numbered from item0; item (t, c) writes partial
+// row pbase + (t * nchunks + c) * tile. Its reduction slots (one per padded
+// query) are numbered from slot0; ldq, and so slot0, are multiples of kBlock.
+struct Span {
+  int model;   // model table index
   int D;
-  int q0;          // first query of the tile (request-relative)
-  int nq;          // valid queries in the tile
-  int j0, j1;      // grid chunk, multiples of kTG
-  long long xoff;  // element offset of the request's query block [D][ldq]
-  int ldq;         // padded query count of the block (multiple of the tile)
+  int tile;    // queries per tile (kBlock * K)
+  int nchunks;
+  int chunk;   // grid points per chunk, a multiple of kTG
+  int nq;      // valid queries
+  int ldq;     // padded query count, a multiple of tile
+  int item0;
+  int slot0;
   int pad;
-  long long poff;  // double offset of this item's partial row (tile doubles)
+  long long xoff;
+  long long pbase;
+  long long out0;  // output offset of the span's first query
+};
+static_assert(sizeof(Span) == 64, "one span is 64 bytes");
+
+// one work item, decoded from its span
+struct Item {
+  int q0;          // first query of the tile (span-relative)
+  int j0, j1;      // grid chunk, multiples of kTG
+  int ldq;
+  long long xoff;
+  long long poff;  // double offset of the item's partial row
 };
 
-struct RedItem {
-  long long pbase;  // partial offset of chunk 0 of this tile
-  int nchunks;
-  int nq;
-  int tile;
-  int pad;
-  long long out;    // output offset of the tile's first query
-};
+// index of the span that holds position v, given each span's first
+// position first(k) (increasing); wave-uniform arguments give scalar code
+template <typename F>
+__device__ __forceinline__ int find_span(int nspans, int v, F first) {
+  int lo = 0, hi = nspans;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first(mid) <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
 template <typename T>
 constexpr int serve_k(int D) {
@@ -191,26 +216,37 @@ __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md
 }
 
 // Persistent work-list kernel: blocks pull items from a counter until the
-// list is exhausted (every block reaches the exit test).
+// list is exhausted (every block reaches the exit test). counts[0] = items,
+// counts[2] = spans. FP32 is held to 128 registers (4 waves per SIMD); the
+// FP64 dimension cases need more and run 3 waves per SIMD.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restrict__ models,
-                                                       const Item* __restrict__ items,
-                                                       const int* __restrict__ counts,
-                                                       unsigned* counter, const T* __restrict__ X,
-                                                       double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
+void serve_kernel(const ModelDesc* __restrict__ models, const Span* __restrict__ spans,
+                  const int* __restrict__ counts, unsigned* counter, const T* __restrict__ X,
+                  double* __restrict__ part) {
   // two staging buffers (double-buffered grid tiles)
   __shared__ __align__(16) unsigned char sbuf[2 * stage_bytes<T>(kMaxD)];
   __shared__ unsigned s_item;
-  const unsigned n = static_cast<unsigned>(counts[0]);
+  const int n = counts[0];
+  const int nspans = counts[2];
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
     __syncthreads();
-    const unsigned it = s_item;
+    const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(s_item));
     __syncthreads();
-    if (it >= n) break;
-    const Item item = items[it];
-    const ModelDesc md = models[item.model];
-    switch (item.D) {
+    if (it >= n || it < 0) break;
+    const Span sp = spans[find_span(nspans, it, [&](int k) { return spans[k].item0; })];
+    const ModelDesc md = models[sp.model];
+    const int local = it - sp.item0;
+    const int t = local / sp.nchunks;
+    Item item;
+    item.q0 = t * sp.tile;
+    item.j0 = (local - t * sp.nchunks) * sp.chunk;
+    item.j1 = min(item.j0 + sp.chunk, md.Npad);
+    item.ldq = sp.ldq;
+    item.xoff = sp.xoff;
+    item.poff = sp.pbase + static_cast<long long>(local) * sp.tile;
+    switch (sp.D) {
 #define SGPP_CASE(DD) \
   case DD:            \
     dispatch_one<T, DD>(item, md, X, part, sbuf); \
@@ -226,19 +262,37 @@ __global__ __launch_bounds__(kBlock) void serve_kernel(const ModelDesc* __restri
   }
 }
 
-// out[tile query i] = sum over chunks in chunk order
-__global__ __launch_bounds__(kBlock) void serve_reduce(const RedItem* __restrict__ red,
+// out[query] = the sum of the query's partials over its model's grid chunks,
+// in chunk order. Each block takes kBlock consecutive reduction slots (all in
+// one span); each thread keeps kRedBatch partial loads in flight and adds
+// them in order. counts[1] = slots, counts[2] = spans.
+constexpr int kRedBatch = 8;
+constexpr int kReduceBlocksPerCU = 8;
+__global__ __launch_bounds__(kBlock) void serve_reduce(const Span* __restrict__ spans,
                                                        const int* __restrict__ counts,
                                                        const double* __restrict__ part,
                                                        double* __restrict__ out) {
-  const int n = counts[1];
-  for (int r = blockIdx.x; r < n; r += gridDim.x) {
-    const RedItem ri = red[r];
-    for (int i = threadIdx.x; i < ri.nq; i += kBlock) {
-      double s = 0.0;
-      for (int c = 0; c < ri.nchunks; ++c) s += part[ri.pbase + static_cast<long long>(c) * ri.tile + i];
-      out[ri.out + i] = s;
+  const int nslots = counts[1];
+  const int nspans = counts[2];
+  for (int s0 = blockIdx.x * kBlock; s0 < nslots; s0 += gridDim.x * kBlock) {
+    const Span sp = spans[find_span(nspans, s0, [&](int k) { return spans[k].slot0; })];
+    const int local = s0 - sp.slot0 + static_cast<int>(threadIdx.x);
+    if (local >= sp.nq) continue;  // padding query
+    const int t = local / sp.tile;
+    const long long step = sp.tile;
+    const double* p = part + sp.pbase + static_cast<long long>(t) * sp.nchunks * step +
+                      (local - t * sp.tile);
+    double sum = 0.0;
+    int c = 0;
+    for (; c + kRedBatch <= sp.nchunks; c += kRedBatch) {
+      double v[kRedBatch];
+#pragma unroll
+      for (int u = 0; u < kRedBatch; ++u) v[u] = p[(c + u) * step];
+#pragma unroll
+      for (int u = 0; u < kRedBatch; ++u) sum += v[u];
     }
+    for (; c < sp.nchunks; ++c) sum += p[c * step];
+    out[sp.out0 + local] = sum;
   }
 }
 
@@ -246,36 +300,26 @@ __global__ void reset_counter(unsigned* counter) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
 }
 
-// Work items and reduction items of one block of q queries of one model
-// (stored [D][ldq] at xoff): ntiles query tiles x nchunks grid chunks;
-// partial rows start at part_base, results go to out_off.
-void add_items(int table_index, int D, size_t Npad, size_t tile, size_t chunk, size_t q,
-               size_t ldq, long long xoff, long long out_off, size_t part_base, Item* items,
-               RedItem* red) {
-  const size_t ntiles = ldq / tile;
-  const size_t nchunks = (Npad + chunk - 1) / chunk;
-  for (size_t tI = 0; tI < ntiles; ++tI) {
-    RedItem& ri = red[tI];
-    ri.pbase = static_cast<long long>(part_base + tI * nchunks * tile);
-    ri.nchunks = static_cast<int>(nchunks);
-    ri.nq = static_cast<int>(std::min<size_t>(tile, q - std::min(q, tI * tile)));
-    ri.tile = static_cast<int>(tile);
-    ri.pad = 0;
-    ri.out = out_off + static_cast<long long>(tI * tile);
-    for (size_t c = 0; c < nchunks; ++c) {
-      Item& it = items[tI * nchunks + c];
-      it.model = table_index;
-      it.D = D;
-      it.q0 = static_cast<int>(tI * tile);
-      it.nq = ri.nq;
-      it.j0 = static_cast<int>(c * chunk);
-      it.j1 = static_cast<int>(std::min<size_t>((c + 1) * chunk, Npad));
-      it.xoff = xoff;
-      it.ldq = static_cast<int>(ldq);
-      it.pad = 0;
-      it.poff = ri.pbase + static_cast<long long>(c * tile);
-    }
-  }
+// Span of q queries of one model (stored [D][ldq] at xoff, ldq a multiple of
+// tile), its items numbered from item0, its slots from slot0, its partials at
+// pbase and its results at out0.
+Span make_span(int table_index, int D, size_t Npad, size_t tile, size_t chunk, size_t q,
+               size_t ldq, long long xoff, size_t item0, size_t slot0, size_t pbase,
+               long long out0) {
+  Span sp{};
+  sp.model = table_index;
+  sp.D = D;
+  sp.tile = static_cast<int>(tile);
+  sp.nchunks = static_cast<int>((Npad + chunk - 1) / chunk);
+  sp.chunk = static_cast<int>(chunk);
+  sp.nq = static_cast<int>(q);
+  sp.ldq = static_cast<int>(ldq);
+  sp.item0 = static_cast<int>(item0);
+  sp.slot0 = static_cast<int>(slot0);
+  sp.xoff = xoff;
+  sp.pbase = static_cast<long long>(pbase);
+  sp.out0 = out0;
+  return sp;
 }
 
 uint64_t splitmix64(uint64_t& s) {
@@ -344,18 +388,16 @@ struct Server::Slot {
   hipEvent_t ev_start = nullptr, ev_done = nullptr;
   hipGraphExec_t graph = nullptr;
   void* dX = nullptr;
-  Item* dItems = nullptr;
-  RedItem* dRed = nullptr;
+  Span* dSpans = nullptr;
   int* dCounts = nullptr;
   unsigned* dCounter = nullptr;
   double* dPart = nullptr;
   double* dOut = nullptr;
   char* hX = nullptr;
-  Item* hItems = nullptr;
-  RedItem* hRed = nullptr;
+  Span* hSpans = nullptr;
   int* hCounts = nullptr;
   double* hOut = nullptr;
-  size_t capX = 0, capItems = 0, capRed = 0, capPart = 0, capOut = 0;
+  size_t capX = 0, capItems = 0, capSpans = 0, capPart = 0, capOut = 0;
   // current batch
   bool busy = false;
   bool inject = false;
@@ -366,11 +408,14 @@ struct Server::Slot {
     size_t q, ldq;
   };
   std::vector<Block> blocks;  // one per model in the batch (padding columns)
-  size_t usedX = 0, nItems = 0, nRed = 0, usedPart = 0, usedOut = 0;
+  size_t usedX = 0, nItems = 0, nSlots = 0, nSpans = 0, usedPart = 0, usedOut = 0;
 };
 
 Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
   if (cfg_.max_inflight < 1 || cfg_.max_inflight > 8) throw input_error("max_inflight in [1, 8]");
+  // item and slot numbers are ints on the device
+  if (cfg_.max_items > (size_t(1) << 30) || cfg_.max_batch_queries > (size_t(1) << 28))
+    throw input_error("max_items <= 2^30 and max_batch_queries <= 2^28");
   check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
   hipDeviceProp_t prop;
   check_hip(hipGetDeviceProperties(&prop, cfg_.device), "hipGetDeviceProperties");
@@ -384,10 +429,9 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     s->capOut = cfg_.max_batch_queries;
     s->capX = cfg_.max_batch_queries * kMaxD + size_t(kMaxD) * 4096 * 64;
     s->capItems = cfg_.max_items;
-    s->capRed = cfg_.max_batch_queries / kBlock + size_t(cfg_.max_requests_per_batch) + 1;
+    s->capSpans = kMaxModels;  // one span per model and batch
     s->capPart = cfg_.max_partial_doubles;
-    const size_t dev_bytes = s->capX * esz + s->capItems * sizeof(Item) +
-                             s->capRed * sizeof(RedItem) + 4 * sizeof(int) +
+    const size_t dev_bytes = s->capX * esz + s->capSpans * sizeof(Span) + 4 * sizeof(int) +
                              s->capPart * sizeof(double) + s->capOut * sizeof(double);
     if (used_bytes_ + dev_bytes > cfg_.memory_budget)
       throw input_error("memory budget too small for the batch buffers");
@@ -396,10 +440,8 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     check_hip(hipEventCreate(&s->ev_start), "hipEventCreate");
     check_hip(hipEventCreate(&s->ev_done), "hipEventCreate");
     check_hip(hipMalloc(&s->dX, s->capX * esz), "hipMalloc(batch X)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dItems), s->capItems * sizeof(Item)),
-              "hipMalloc(items)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dRed), s->capRed * sizeof(RedItem)),
-              "hipMalloc(red)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dSpans), s->capSpans * sizeof(Span)),
+              "hipMalloc(spans)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounts), 4 * sizeof(int)),
               "hipMalloc(counts)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounter), sizeof(unsigned)),
@@ -409,9 +451,7 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dOut), s->capOut * sizeof(double)),
               "hipMalloc(out)");
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hX), s->capX * esz), "hipHostMalloc");
-    check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hItems), s->capItems * sizeof(Item)),
-              "hipHostMalloc");
-    check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hRed), s->capRed * sizeof(RedItem)),
+    check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hSpans), s->capSpans * sizeof(Span)),
               "hipHostMalloc");
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hCounts), 4 * sizeof(int)),
               "hipHostMalloc");
@@ -427,14 +467,14 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
       hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s->stream, s->dCounter);
       if (cfg_.precision == Precision::FP64)
         hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s->stream,
-                           static_cast<const ModelDesc*>(model_table_), s->dItems, s->dCounts,
+                           static_cast<const ModelDesc*>(model_table_), s->dSpans, s->dCounts,
                            s->dCounter, static_cast<const double*>(s->dX), s->dPart);
       else
         hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, s->stream,
-                           static_cast<const ModelDesc*>(model_table_), s->dItems, s->dCounts,
+                           static_cast<const ModelDesc*>(model_table_), s->dSpans, s->dCounts,
                            s->dCounter, static_cast<const float*>(s->dX), s->dPart);
-      hipLaunchKernelGGL(serve_reduce, dim3(2 * num_cus_), dim3(kBlock), 0, s->stream, s->dRed,
-                         s->dCounts, s->dPart, s->dOut);
+      hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
+                         s->stream, s->dSpans, s->dCounts, s->dPart, s->dOut);
       check_hip(hipStreamEndCapture(s->stream, &g), "hipStreamEndCapture");
       check_hip(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
       check_hip(hipGraphDestroy(g), "hipGraphDestroy");
@@ -458,10 +498,10 @@ Server::~Server() {
   for (auto& s : slots_) {
     (void)hipStreamSynchronize(s->stream);
     if (s->graph) (void)hipGraphExecDestroy(s->graph);
-    void* dev[] = {s->dX, s->dItems, s->dRed, s->dCounts, s->dCounter, s->dPart, s->dOut};
+    void* dev[] = {s->dX, s->dSpans, s->dCounts, s->dCounter, s->dPart, s->dOut};
     for (void* p : dev)
       if (p) (void)hipFree(p);
-    void* host[] = {s->hX, s->hItems, s->hRed, s->hCounts, s->hOut};
+    void* host[] = {s->hX, s->hSpans, s->hCounts, s->hOut};
     for (void* p : host)
       if (p) (void)hipHostFree(p);
     (void)hipEventDestroy(s->ev_start);
@@ -732,7 +772,7 @@ void Server::fail_pieces(Slot& s, const char* why) {
 // When the rest of the front request does not fit, its longest prefix that
 // does is placed and the request stays at the front for the next batch.
 bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
-  s.usedX = s.nItems = s.nRed = s.usedPart = s.usedOut = 0;
+  s.usedX = s.nItems = s.nSlots = s.nSpans = s.usedPart = s.usedOut = 0;
   s.blocks.clear();
   const double t = now_s();
   struct Group {
@@ -743,15 +783,13 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
   std::vector<Group> groups;
   std::map<int, size_t> gidx;
   // resources of a block of q queries
-  auto need = [](const Group& g, size_t q, size_t* xel, size_t* items, size_t* part,
-                 size_t* red) {
+  auto need = [](const Group& g, size_t q, size_t* xel, size_t* items, size_t* part) {
     const size_t ntiles = (q + g.tile - 1) / g.tile;
     *xel = static_cast<size_t>(g.m->D) * ntiles * g.tile;
     *items = ntiles * g.nchunks;
     *part = *items * g.tile;
-    *red = ntiles;
   };
-  size_t useX = 0, useItems = 0, usePart = 0, useRed = 0, useOut = 0;
+  size_t useX = 0, useItems = 0, usePart = 0, useOut = 0;
   while (!queue_.empty() && taken.size() < static_cast<size_t>(cfg_.max_requests_per_batch)) {
     std::shared_ptr<Request> r = queue_.front();
     if (r->status != REQ_QUEUED) {  // cancelled or failed meanwhile
@@ -767,23 +805,22 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
     Group probe{&m, static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D), chunk_for(m), 0,
                 0, {}};
     probe.nchunks = (m.Npad + probe.chunk - 1) / probe.chunk;
-    size_t x1, i1, p1, r1;
-    need(probe, 1, &x1, &i1, &p1, &r1);
-    if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX || r1 > s.capRed) {
+    size_t x1, i1, p1;
+    need(probe, 1, &x1, &i1, &p1);
+    if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX) {
       queue_.pop_front();
       fail_request(r, REQ_FAILED_BUDGET, "one query tile of this model exceeds the batch buffers");
       continue;
     }
     auto gi = gidx.find(r->model);
     const Group& g = gi == gidx.end() ? probe : groups[gi->second];
-    size_t x0 = 0, i0 = 0, p0 = 0, r0 = 0;
-    if (gi != gidx.end()) need(g, g.q, &x0, &i0, &p0, &r0);
+    size_t x0 = 0, i0 = 0, p0 = 0;
+    if (gi != gidx.end()) need(g, g.q, &x0, &i0, &p0);
     // does a piece of n more queries fit? (monotone in n)
     auto fits = [&](size_t n) {
-      need(g, g.q + n, &x1, &i1, &p1, &r1);
+      need(g, g.q + n, &x1, &i1, &p1);
       return useX + x1 - x0 <= s.capX && useItems + i1 - i0 <= s.capItems &&
-             usePart + p1 - p0 <= s.capPart && useRed + r1 - r0 <= s.capRed &&
-             useOut + n <= s.capOut;
+             usePart + p1 - p0 <= s.capPart && useOut + n <= s.capOut;
     };
     const size_t rest = r->q - r->next;
     size_t n = rest;
@@ -796,11 +833,10 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
       n = lo;
       if (n == 0) break;  // full: the rest waits for the next slot (FIFO)
     }
-    need(g, g.q + n, &x1, &i1, &p1, &r1);
+    need(g, g.q + n, &x1, &i1, &p1);
     useX += x1 - x0;
     useItems += i1 - i0;
     usePart += p1 - p0;
-    useRed += r1 - r0;
     useOut += n;
     if (gi == gidx.end()) {
       gidx[r->model] = groups.size();
@@ -833,11 +869,11 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
       pc.ldq = ldq;
       pc.out_off = out0 + static_cast<long long>(pc.qoff);
     }
-    add_items(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q, ldq, xoff, out0,
-              s.usedPart, s.hItems + s.nItems, s.hRed + s.nRed);
+    s.hSpans[s.nSpans++] = make_span(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q,
+                                     ldq, xoff, s.nItems, s.nSlots, s.usedPart, out0);
     s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq});
     s.nItems += ntiles * g.nchunks;
-    s.nRed += ntiles;
+    s.nSlots += ldq;
     s.usedPart += ntiles * g.nchunks * g.tile;
     s.usedX += static_cast<size_t>(g.m->D) * ldq;
     s.usedOut += g.q;
@@ -859,16 +895,14 @@ void Server::launch_batch(Slot& s) {
     for (int d = 0; d < b.D; ++d)
       std::memset(s.hX + (b.xoff + d * b.ldq + b.q) * esz, 0, (b.ldq - b.q) * esz);
   s.hCounts[0] = static_cast<int>(s.nItems);
-  s.hCounts[1] = static_cast<int>(s.nRed);
-  s.hCounts[2] = s.hCounts[3] = 0;
+  s.hCounts[1] = static_cast<int>(s.nSlots);
+  s.hCounts[2] = static_cast<int>(s.nSpans);
+  s.hCounts[3] = 0;
   check_hip(hipEventRecord(s.ev_start, s.stream), "hipEventRecord");
   check_hip(hipMemcpyAsync(s.dX, s.hX, s.usedX * esz, hipMemcpyHostToDevice, s.stream), "H2D X");
-  check_hip(hipMemcpyAsync(s.dItems, s.hItems, s.nItems * sizeof(Item), hipMemcpyHostToDevice,
+  check_hip(hipMemcpyAsync(s.dSpans, s.hSpans, s.nSpans * sizeof(Span), hipMemcpyHostToDevice,
                            s.stream),
-            "H2D items");
-  check_hip(hipMemcpyAsync(s.dRed, s.hRed, s.nRed * sizeof(RedItem), hipMemcpyHostToDevice,
-                           s.stream),
-            "H2D red");
+            "H2D spans");
   check_hip(hipMemcpyAsync(s.dCounts, s.hCounts, 4 * sizeof(int), hipMemcpyHostToDevice,
                            s.stream),
             "H2D counts");
@@ -879,14 +913,14 @@ void Server::launch_batch(Slot& s) {
     hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s.stream, s.dCounter);
     if (cfg_.precision == Precision::FP64)
       hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s.stream,
-                         static_cast<const ModelDesc*>(model_table_), s.dItems, s.dCounts,
+                         static_cast<const ModelDesc*>(model_table_), s.dSpans, s.dCounts,
                          s.dCounter, static_cast<const double*>(s.dX), s.dPart);
     else
       hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, s.stream,
-                         static_cast<const ModelDesc*>(model_table_), s.dItems, s.dCounts,
+                         static_cast<const ModelDesc*>(model_table_), s.dSpans, s.dCounts,
                          s.dCounter, static_cast<const float*>(s.dX), s.dPart);
-    hipLaunchKernelGGL(serve_reduce, dim3(2 * num_cus_), dim3(kBlock), 0, s.stream, s.dRed,
-                       s.dCounts, s.dPart, s.dOut);
+    hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
+                       s.stream, s.dSpans, s.dCounts, s.dPart, s.dOut);
     check_hip(hipGetLastError(), "serve launch");
   }
   check_hip(hipMemcpyAsync(s.hOut, s.dOut, s.usedOut * sizeof(double), hipMemcpyDeviceToHost,
@@ -955,7 +989,8 @@ bool Server::tuned_shape(Basis basis, int dim, TunedShape* out) {
 
 // Times every candidate grid chunk (grid points per work item) on the model
 // with synthetic uniform queries, alone on its own stream, and keeps the
-// fastest with its cost per (query x grid point). The load follows the
+// fastest with its cost per (query x grid point); the time covers the
+// work-list kernel and the reduction of its partials. The load follows the
 // server's batching: one request of kTuneQueries when every launch holds one
 // request (latency of a lone request), kTuneBatchCopies times as many when
 // batches pack many requests (throughput of a saturated device). The result
@@ -983,19 +1018,18 @@ void Server::tune(Model& m) {
     if (static_cast<size_t>(c) < m.Npad && ntiles * ((m.Npad + c - 1) / c) * tile <= kTunePartDoubles)
       cands.push_back(c);
   cands.push_back(static_cast<int>(m.Npad));
-  size_t max_items = 0;
-  for (int c : cands) max_items = std::max(max_items, ntiles * ((m.Npad + c - 1) / c));
   check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   void* dX = nullptr;
-  Item* dItems = nullptr;
+  Span* dSpans = nullptr;
   int* dCounts = nullptr;
   unsigned* dCounter = nullptr;
   double* dPart = nullptr;
+  double* dOut = nullptr;
   auto release = [&] {
     if (stream) (void)hipStreamSynchronize(stream);
-    void* p[] = {dX, dItems, dCounts, dCounter, dPart};
+    void* p[] = {dX, dSpans, dCounts, dCounter, dPart, dOut};
     for (void* x : p)
       if (x) (void)hipFree(x);
     if (e0) (void)hipEventDestroy(e0);
@@ -1008,8 +1042,9 @@ void Server::tune(Model& m) {
     check_hip(hipEventCreate(&e0), "hipEventCreate");
     check_hip(hipEventCreate(&e1), "hipEventCreate");
     check_hip(hipMalloc(&dX, hx.size()), "hipMalloc(tune X)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&dItems), max_items * sizeof(Item)),
-              "hipMalloc(tune items)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dSpans), sizeof(Span)), "hipMalloc(tune span)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dOut), nq * sizeof(double)),
+              "hipMalloc(tune out)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), 4 * sizeof(int)), "hipMalloc");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), sizeof(unsigned)), "hipMalloc");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dPart),
@@ -1020,13 +1055,10 @@ void Server::tune(Model& m) {
     for (int c : cands) {
       const size_t nchunks = (m.Npad + c - 1) / c;
       const size_t nitems = ntiles * nchunks;
-      std::vector<Item> items(nitems);
-      std::vector<RedItem> red(ntiles);
-      add_items(m.table_index, m.D, m.Npad, tile, static_cast<size_t>(c), nq, nq, 0, 0, 0,
-                items.data(), red.data());
-      const int counts[4] = {static_cast<int>(nitems), 0, 0, 0};
-      check_hip(hipMemcpy(dItems, items.data(), nitems * sizeof(Item), hipMemcpyHostToDevice),
-                "upload tune items");
+      const Span sp = make_span(m.table_index, m.D, m.Npad, tile, static_cast<size_t>(c), nq, nq,
+                                0, 0, 0, 0, 0);
+      const int counts[4] = {static_cast<int>(nitems), static_cast<int>(nq), 1, 0};
+      check_hip(hipMemcpy(dSpans, &sp, sizeof(Span), hipMemcpyHostToDevice), "upload tune span");
       check_hip(hipMemcpy(dCounts, counts, sizeof(counts), hipMemcpyHostToDevice),
                 "upload tune counts");
       float tmin = -1.0f;
@@ -1036,12 +1068,14 @@ void Server::tune(Model& m) {
         hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, dCounter);
         if (f64)
           hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
-                             static_cast<const ModelDesc*>(model_table_), dItems, dCounts,
+                             static_cast<const ModelDesc*>(model_table_), dSpans, dCounts,
                              dCounter, static_cast<const double*>(dX), dPart);
         else
           hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
-                             static_cast<const ModelDesc*>(model_table_), dItems, dCounts,
+                             static_cast<const ModelDesc*>(model_table_), dSpans, dCounts,
                              dCounter, static_cast<const float*>(dX), dPart);
+        hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
+                           stream, dSpans, dCounts, dPart, dOut);
         check_hip(hipGetLastError(), "tune launch");
         check_hip(hipEventRecord(e1, stream), "hipEventRecord");
         check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
