@@ -201,8 +201,8 @@ typedef struct sgpp_hip_server_config {
   int device;
   int precision;                /* SGPP_HIP_PREC_* of the served products */
   size_t memory_budget;         /* bytes for models + batch buffers; 0 = 64 GiB */
-  size_t max_batch_queries;     /* queries per batch; 0 = 4 Mi */
-  size_t max_items;             /* work items per batch; 0 = 256 Ki */
+  size_t max_batch_queries;     /* queries per batch; 0 = 4 Mi, at most 2^28 */
+  size_t max_items;             /* work items per batch; 0 = 256 Ki, at most 2^30 */
   size_t max_partial_doubles;   /* partial-sum capacity per batch; 0 = 128 Mi */
   int max_inflight;             /* batches on the GPU at once; 0 = 2 */
   int max_requests_per_batch;   /* 1 = one launch per request; 0 = 65536 */
@@ -224,7 +224,7 @@ enum {
   SGPP_HIP_REQ_QUEUED = 0,
   SGPP_HIP_REQ_RUNNING = 1,
   SGPP_HIP_REQ_DONE = 2,
-  SGPP_HIP_REQ_FAILED_BUDGET = 3,
+  SGPP_HIP_REQ_FAILED_BUDGET = 3,  /* one query tile of the model exceeds a batch's buffers */
   SGPP_HIP_REQ_FAILED_TIMEOUT = 4,
   SGPP_HIP_REQ_CANCELLED = 5,
   SGPP_HIP_REQ_FAILED_DEVICE = 6
