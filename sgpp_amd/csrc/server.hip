@@ -27,6 +27,7 @@ struct ModelDesc {
   const void* LI;         // tile-major (L, I): [Npad/64][D][64][2]
   const void* A;          // [Npad] alpha * scale, zero padded
   const uint32_t* qmask;  // [Npad/4] dimensions evaluated per 4 consecutive points
+  const uint32_t* umask;  // [Npad/J] dimensions whose (L, I) a sub-tile's J points share
   int D;
   int Npad;               // multiple of kTG
 };
@@ -91,14 +92,17 @@ constexpr int serve_j(int D) {
 }
 
 // LDS staging buffer of one 64-point grid tile: weights A (kTG values),
-// per-4-point masks (kTG/4 words), then the tile-major (L, I) pairs
+// per-4-point masks (kTG/4 words), shared-dimension masks per J-point
+// sub-tile (kTG/J <= kTG/4 words), then the tile-major (L, I) pairs
 // (D*kTG*2 values); every part starts on a 16-byte boundary.
 template <typename T>
 constexpr int stage_a_off() { return 0; }
 template <typename T>
 constexpr int stage_m_off() { return kTG * static_cast<int>(sizeof(T)); }
 template <typename T>
-constexpr int stage_li_off() { return stage_m_off<T>() + kTG; }
+constexpr int stage_u_off() { return stage_m_off<T>() + kTG; }
+template <typename T>
+constexpr int stage_li_off() { return stage_u_off<T>() + kTG; }
 template <typename T>
 constexpr int stage_bytes(int D) { return stage_li_off<T>() + kTG * D * 2 * static_cast<int>(sizeof(T)); }
 
@@ -111,8 +115,9 @@ __device__ __forceinline__ void copy_to_lds(const uint4* __restrict__ src, uint4
     if (c0 + lane < n) __builtin_amdgcn_global_load_lds(src + c0 + lane, dst + c0, 16, 0, 0);
 }
 
-// Starts the copy of grid tile j0 of model md into the staging buffer buf.
-template <int D, typename T>
+// Starts the copy of grid tile j0 of model md (sub-tiles of J points) into the
+// staging buffer buf.
+template <int D, int J, typename T>
 __device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned char* buf) {
   const int wave = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x >> 6));
   const int lane = threadIdx.x & 63;
@@ -123,6 +128,8 @@ __device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned
               reinterpret_cast<uint4*>(buf + stage_a_off<T>()), kTG * sizeof(T) / 16, wave, lane);
   copy_to_lds(reinterpret_cast<const uint4*>(M + j0), reinterpret_cast<uint4*>(buf + stage_m_off<T>()),
               kTG / 16, wave, lane);
+  copy_to_lds(reinterpret_cast<const uint4*>(md.umask + j0 / J),
+              reinterpret_cast<uint4*>(buf + stage_u_off<T>()), kTG / J / 4, wave, lane);
   copy_to_lds(reinterpret_cast<const uint4*>(LI + static_cast<long long>(j0) * D * 2 * sizeof(T)),
               reinterpret_cast<uint4*>(buf + stage_li_off<T>()), kTG * D * 2 * sizeof(T) / 16,
               wave, lane);
@@ -150,14 +157,15 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
   int cur = 0;
-  stage_tile<D, T>(md, item.j0, sbuf);
+  stage_tile<D, J, T>(md, item.j0, sbuf);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
   for (int j0 = item.j0; j0 < item.j1; j0 += kTG) {
-    if (j0 + kTG < item.j1) stage_tile<D, T>(md, j0 + kTG, sbuf + (cur ^ 1) * bufbytes);
+    if (j0 + kTG < item.j1) stage_tile<D, J, T>(md, j0 + kTG, sbuf + (cur ^ 1) * bufbytes);
     const unsigned char* buf = sbuf + cur * bufbytes;
     const T* sA = reinterpret_cast<const T*>(buf + stage_a_off<T>());
     const uint32_t* sM = reinterpret_cast<const uint32_t*>(buf + stage_m_off<T>());
+    const uint32_t* sU = reinterpret_cast<const uint32_t*>(buf + stage_u_off<T>());
     const T* sLI = reinterpret_cast<const T*>(buf + stage_li_off<T>());
     T acc[K];
 #pragma unroll
@@ -167,14 +175,24 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 #pragma unroll
       for (int c = 0; c < J / 4; ++c) mask |= sM[j / 4 + c];
       mask = __builtin_amdgcn_readfirstlane(mask);
+      const uint32_t shared = __builtin_amdgcn_readfirstlane(sU[j / J]);
       T p[J][K];
+      T g[K];  // product of the factors the J points share
 #pragma unroll
-      for (int q = 0; q < J; ++q)
+      for (int k = 0; k < K; ++k) {
+        g[k] = T(1);
 #pragma unroll
-        for (int k = 0; k < K; ++k) p[q][k] = T(1);
+        for (int q = 0; q < J; ++q) p[q][k] = T(1);
+      }
 #pragma unroll
       for (int d = 0; d < D; ++d) {
         if (!((mask >> d) & 1u)) continue;
+        if ((shared >> d) & 1u) {  // one factor per query for the whole sub-tile
+          const T L = sLI[(d * kTG + j) * 2], I = sLI[(d * kTG + j) * 2 + 1];
+#pragma unroll
+          for (int k = 0; k < K; ++k) g[k] *= clamp01(T(1) - absT(fmaT(L, x[k][d], -I)));
+          continue;
+        }
         T li[2 * J];
 #pragma unroll
         for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
@@ -186,12 +204,17 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
             p[q][k] = clamp01(fmaT(-p[q][k], absT(u), p[q][k]));
           }
       }
+      T sub[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) sub[k] = T(0);
 #pragma unroll
       for (int q = 0; q < J; ++q) {
         const T a = sA[j + q];
 #pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = fmaT(a, p[q][k], acc[k]);
+        for (int k = 0; k < K; ++k) sub[k] = fmaT(a, p[q][k], sub[k]);
       }
+#pragma unroll
+      for (int k = 0; k < K; ++k) acc[k] = fmaT(g[k], sub[k], acc[k]);
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
@@ -207,6 +230,10 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 
 int serve_k_rt(Precision p, int D) {
   return p == Precision::FP64 ? serve_k<double>(D) : serve_k<float>(D);
+}
+
+int serve_j_rt(Precision p, int D) {
+  return p == Precision::FP64 ? serve_j<double>(D) : serve_j<float>(D);
 }
 
 template <typename T, int D>
@@ -351,13 +378,14 @@ struct Server::Model {
   void* LI = nullptr;
   void* A = nullptr;
   void* pm = nullptr;  // [Npad/4] dimensions evaluated per 4 consecutive points
+  void* um = nullptr;  // [Npad/J] dimensions whose (L, I) the J points of a sub-tile share
   size_t bytes = 0;
   int active = 0;  // requests queued or running
   void free_device() {
-    void* p[] = {LI, A, pm};
+    void* p[] = {LI, A, pm, um};
     for (void* q : p)
       if (q) (void)hipFree(q);
-    LI = A = pm = nullptr;
+    LI = A = pm = um = nullptr;
   }
 };
 
@@ -516,30 +544,37 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
                         const uint32_t* index, const double* alpha) {
   if (dim < 1 || dim > static_cast<size_t>(kMaxD)) throw input_error("model dimension in [1, 20]");
   if (N == 0) throw input_error("empty model");
-  // ModLinear: points sorted by their set of non-constant dimensions (level
-  // above 1), so that tiles share masks; the model's output is a sum over all
-  // points, so the order changes nothing but the summation order
+  // Points are sorted by their set of dimensions above level 1, then by their
+  // encoded (L, I) pairs, dimension 0 first. Consecutive points then share
+  // whole dimensions: ModLinear level-1 factors are constant and skipped per
+  // 4 points; a dimension whose (L, I) all J points of a sub-tile share is
+  // evaluated once per query. The model's output is a sum over all points,
+  // so the order changes nothing but the summation order.
   const bool skip = basis == Basis::ModLinear && dim >= 2;
-  std::vector<uint32_t> pmask(N);
+  std::vector<double> lip0, sc0;
+  encode_grid(basis, dim, N, level, index, lip0, sc0);
+  std::vector<uint32_t> cls(N), pmask(N);
   for (size_t j = 0; j < N; ++j) {
-    uint32_t mk = 0;
+    uint32_t c = 0;
     for (size_t d = 0; d < dim; ++d)
-      if (!skip || level[j * dim + d] > 1) mk |= 1u << d;
-    pmask[j] = mk;
+      if (level[j * dim + d] > 1) c |= 1u << d;
+    cls[j] = c;
+    pmask[j] = skip ? c : (1u << dim) - 1u;
   }
   std::vector<size_t> order(N);
   for (size_t j = 0; j < N; ++j) order[j] = j;
-  if (skip)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](size_t a, size_t b) { return pmask[a] < pmask[b]; });
-  std::vector<uint32_t> lv(N * dim), ix(N * dim);
-  for (size_t s2 = 0; s2 < N; ++s2)
-    for (size_t d = 0; d < dim; ++d) {
-      lv[s2 * dim + d] = level[order[s2] * dim + d];
-      ix[s2 * dim + d] = index[order[s2] * dim + d];
-    }
-  std::vector<double> lip, sc;
-  encode_grid(basis, dim, N, lv.data(), ix.data(), lip, sc);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    if (cls[a] != cls[b]) return cls[a] < cls[b];
+    for (size_t e = 0; e < 2 * dim; ++e)
+      if (lip0[a * 2 * dim + e] != lip0[b * 2 * dim + e])
+        return lip0[a * 2 * dim + e] < lip0[b * 2 * dim + e];
+    return false;
+  });
+  std::vector<double> lip(N * dim * 2), sc(N);
+  for (size_t s2 = 0; s2 < N; ++s2) {
+    for (size_t e = 0; e < 2 * dim; ++e) lip[s2 * 2 * dim + e] = lip0[order[s2] * 2 * dim + e];
+    sc[s2] = sc0[order[s2]];
+  }
   auto m = std::make_shared<Model>();
   m->basis = basis;
   m->D = static_cast<int>(dim);
@@ -547,9 +582,15 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   m->Npad = ((N + kTG - 1) / kTG) * kTG;
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
-  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + (m->Npad / 4) * sizeof(uint32_t);
+  const size_t J = static_cast<size_t>(serve_j_rt(cfg_.precision, m->D));
+  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + (m->Npad / 4 + m->Npad / J) * sizeof(uint32_t);
   std::vector<double> A(m->Npad, 0.0);
   for (size_t s2 = 0; s2 < N; ++s2) A[s2] = alpha[order[s2]] * sc[s2];
+  // pairs as the kernel sees them (rounded to the compute type)
+  auto pair_at = [&](size_t j, size_t d, int c) {
+    const double v = lip[(j * dim + d) * 2 + c];
+    return f64 ? v : static_cast<double>(static_cast<float>(v));
+  };
   // tile-major (L, I); padded points (0, 0) evaluate to 1 with weight 0
   std::vector<double> li(m->Npad * dim * 2, 0.0);
   for (size_t j = 0; j < N; ++j)
@@ -561,6 +602,22 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   // padded points keep mask 0: nothing evaluated, weight 0
   std::vector<uint32_t> qm(m->Npad / 4, 0u);
   for (size_t s2 = 0; s2 < N; ++s2) qm[s2 / 4] |= pmask[order[s2]];
+  // per sub-tile of J points: dimensions in which every real point has the
+  // (L, I) of the sub-tile's first point (padded points carry weight 0, so
+  // their factor does not matter; a sub-tile of padding only is never
+  // evaluated)
+  std::vector<uint32_t> um(m->Npad / J, 0u);
+  for (size_t t = 0; t * J < N; ++t) {
+    const size_t j0 = t * J, j1 = std::min(N, j0 + J);
+    uint32_t u = 0;
+    for (size_t d = 0; d < dim; ++d) {
+      bool same = true;
+      for (size_t j = j0 + 1; j < j1 && same; ++j)
+        same = pair_at(j, d, 0) == pair_at(j0, d, 0) && pair_at(j, d, 1) == pair_at(j0, d, 1);
+      if (same) u |= 1u << d;
+    }
+    um[t] = u;
+  }
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (used_bytes_ + m->bytes > cfg_.memory_budget)
@@ -589,6 +646,9 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     check_hip(hipMalloc(&m->pm, qm.size() * sizeof(uint32_t)), "hipMalloc(model masks)");
     check_hip(hipMemcpy(m->pm, qm.data(), qm.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
               "upload masks");
+    check_hip(hipMalloc(&m->um, um.size() * sizeof(uint32_t)), "hipMalloc(model shared masks)");
+    check_hip(hipMemcpy(m->um, um.data(), um.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "upload shared masks");
     if (f64) {
       check_hip(hipMemcpy(m->LI, li.data(), li.size() * 8, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, A.data(), A.size() * 8, hipMemcpyHostToDevice), "upload A");
@@ -597,8 +657,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       check_hip(hipMemcpy(m->LI, l32.data(), l32.size() * 4, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, a32.data(), a32.size() * 4, hipMemcpyHostToDevice), "upload A");
     }
-    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->pm), m->D,
-                 static_cast<int>(m->Npad)};
+    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->pm),
+                 static_cast<const uint32_t*>(m->um), m->D, static_cast<int>(m->Npad)};
     check_hip(hipMemcpy(static_cast<ModelDesc*>(model_table_) + m->table_index, &md,
                         sizeof(ModelDesc), hipMemcpyHostToDevice),
               "upload model descriptor");
