@@ -45,6 +45,10 @@ def parse():
                     help="serve with the untuned default tile shapes")
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served workload (requests/s)")
+    ap.add_argument("--serve-no-baseline", action="store_true",
+                    help="serve only; skip the one-launch-per-request FP64 baseline")
+    ap.add_argument("--serve-autotune-cache", default=None,
+                    help="file of tuned tile shapes, read and updated by both servers")
     ap.add_argument("--solve", action="store_true",
                     help="run the regression-solve workload (FP64 CG vs mixed precision) instead")
     ap.add_argument("--inner-epsilon", type=float, default=1e-5)
@@ -213,11 +217,14 @@ def serve_bench(args):
     flops = float(sum(6 * D[m] * N[m] * q for m, q in zip(c["model"], c["q"])))
     runs = {}
     outputs = {}
-    for name, kw in (("served", dict(precision=capi.PREC_FP32, max_inflight=2)),
-                     ("one_launch_per_request_fp64",
-                      dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
-                           use_graph=False))):
+    configs = [("served", dict(precision=capi.PREC_FP32, max_inflight=2))]
+    if not args.serve_no_baseline:
+        configs.append(("one_launch_per_request_fp64",
+                        dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
+                             use_graph=False)))
+    for name, kw in configs:
         kw["autotune"] = not args.no_autotune
+        kw["autotune_cache"] = args.serve_autotune_cache
         t = time.perf_counter()
         srv = capi.Server(**kw)
         ids = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
@@ -234,10 +241,10 @@ def serve_bench(args):
         runs[name] = res
         outputs[name] = outs
         srv.close()
-    errs = [np.linalg.norm(outputs["served"][k] - outputs["one_launch_per_request_fp64"][k]) /
-            max(np.linalg.norm(outputs["one_launch_per_request_fp64"][k]), 1e-300)
-            for k in outputs["served"] if k in outputs["one_launch_per_request_fp64"]]
-    base = runs["one_launch_per_request_fp64"]
+    ref = outputs.get("one_launch_per_request_fp64", {})
+    errs = [np.linalg.norm(outputs["served"][k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
+            for k in outputs["served"] if k in ref]
+    base = runs.get("one_launch_per_request_fp64")
     line = {
         "metric": "served evals/s + p50/p99 latency",
         "value": runs["served"]["evals_per_s"],
@@ -256,8 +263,9 @@ def serve_bench(args):
                    "offered_tflops_6d": flops / c["duration"] / 1e12},
         "served": runs["served"],
         "baseline": base,
-        "evals_ratio_vs_baseline": runs["served"]["evals_per_s"] / base["evals_per_s"],
-        "p99_ratio_vs_baseline": runs["served"]["p99_ms"] / base["p99_ms"],
+        "evals_ratio_vs_baseline": runs["served"]["evals_per_s"] / base["evals_per_s"]
+        if base else None,
+        "p99_ratio_vs_baseline": runs["served"]["p99_ms"] / base["p99_ms"] if base else None,
         "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
         "setup_s": gen_s,
     }

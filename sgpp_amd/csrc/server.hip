@@ -138,10 +138,12 @@ __device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned
 // one work item: a tile of kBlock*K queries of one request against one grid
 // chunk. The chunk's 64-point grid tiles are staged in LDS, double-buffered:
 // the copy of tile t+1 runs while tile t is evaluated, one barrier per tile.
-// A tile is walked in sub-tiles of J points; per (sub-tile, dimension) one
-// wave-uniform test of the sub-tile's mask decides whether the dimension is
-// evaluated (ModLinear level-1 factors are exactly 1; Linear models carry
-// full masks).
+// A tile is walked in sub-tiles of J points; per (sub-tile, dimension)
+// wave-uniform tests of the sub-tile's masks decide whether the dimension is
+// skipped (ModLinear level-1 factors are exactly 1; Linear models carry full
+// masks), evaluated once per query because all J points share its (L, I)
+// (the factor then scales the sub-tile's weighted sum), or evaluated per
+// point.
 template <int D, int K, int J, typename T>
 __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md, const T* X,
                                            double* part, unsigned char* sbuf, int bufbytes) {

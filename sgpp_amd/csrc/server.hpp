@@ -15,7 +15,9 @@
 //     by a reduction kernel; both are replayed from a captured hipGraph;
 //   * the kernel stages 64-point grid tiles in LDS and evaluates them in
 //     sub-tiles, skipping per sub-tile the dimensions in which every point is
-//     constant (ModLinear level 1; models are stored sorted by that set);
+//     constant (ModLinear level 1) and evaluating once per query those in
+//     which every point has the same (L, I); models are stored sorted by
+//     their refined-dimension sets and (L, I) so that sub-tiles share both;
 //   * the queries of all requests of a batch for one model form one block,
 //     tiled together (a request pays for at most one partial query tile per
 //     model and batch); the grid chunk per work item comes from an autotuner
