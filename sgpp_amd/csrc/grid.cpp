@@ -132,21 +132,34 @@ std::string HashGridStorage::serialize() const {
   return os.str();
 }
 
+// HashGridStorage::parseGridDescription (HashGridStorage.cpp:605-684) and
+// HashGridPoint(std::istream&, version) (HashGridPoint.cpp:52-89), versions 1
+// to 9 (SerializationVersion.hpp:11-24): versions 1-2 carry no bounding box,
+// 3-4 one "left right bLeft bRight" per dimension, 5 and later a stretching
+// flag first (only 0, a bounding box, is supported); points carry a leaf flag
+// from version 2 on (not in 4) and a point-distribution field in 6-8.
+// Versions 1 and 4 recompute the leaf flags. Only the unit box is accepted.
 void HashGridStorage::unserialize(const std::string& text) {
   std::istringstream is(text);
   int version = 0;
   size_t dim = 0, n = 0;
   is >> version >> dim >> n;
-  if (!is || version < 9) throw generation_exception("unsupported grid serialization version");
+  if (!is || version < 1 || (version > 9 && version != 4))
+    throw generation_exception("unsupported grid serialization version");
   if (dim != dim_) throw generation_exception("grid dimension mismatch in serialization");
-  int stretch = 0;
-  is >> stretch;
-  if (stretch != 0) throw generation_exception("stretched grids are not supported");
-  for (size_t d = 0; d < dim_; ++d) {
-    double lft, rgt;
-    int bl, br;
-    is >> lft >> rgt >> bl >> br;
-    if (lft != 0.0 || rgt != 1.0) throw generation_exception("only the unit bounding box is supported");
+  if (version >= 5) {
+    int stretch = 0;
+    is >> stretch;
+    if (stretch != 0) throw generation_exception("stretched grids are not supported");
+  }
+  if (version >= 3) {
+    for (size_t d = 0; d < dim_; ++d) {
+      double lft, rgt;
+      int bl, br;
+      is >> lft >> rgt >> bl >> br;
+      if (lft != 0.0 || rgt != 1.0)
+        throw generation_exception("only the unit bounding box is supported");
+    }
   }
   clear();
   HashGridPoint p(dim_);
@@ -161,10 +174,41 @@ void HashGridStorage::unserialize(const std::string& text) {
       p.push(d, l, i);
     }
     int leaf = 0;
-    is >> leaf;
+    if (version >= 2 && version != 4) is >> leaf;
+    if (version >= 6 && version <= 8) {
+      size_t distribution;
+      is >> distribution;
+    }
     p.setLeaf(leaf != 0);
     if (!is) throw generation_exception("truncated grid serialization");
     insert(p);
+  }
+  if (version == 1 || version == 4) recalcLeafProperty();
+}
+
+// HashGridStorage::recalcLeafProperty (HashGridStorage.cpp:358-397): a point
+// is a leaf when neither child in any dimension (for level 0: the level-1
+// point) is in the grid.
+void HashGridStorage::recalcLeafProperty() {
+  for (HashGridPoint* stored : list_) {
+    HashGridPoint q(*stored);
+    bool leaf = true;
+    for (size_t d = 0; d < dim_; ++d) {
+      level_t l;
+      index_t i;
+      q.get(d, l, i);
+      if (l > 0) {
+        q.set(d, l + 1, 2 * i - 1);
+        leaf = leaf && !isContaining(q);
+        q.set(d, l + 1, 2 * i + 1);
+        leaf = leaf && !isContaining(q);
+      } else {
+        q.set(d, 1, 1);
+        leaf = leaf && !isContaining(q);
+      }
+      q.set(d, l, i);
+    }
+    stored->setLeaf(leaf);
   }
 }
 

@@ -207,9 +207,10 @@ class HashGridStorage {
   void getCoordinateArrays(std::vector<double>& coords) const;
   size_t getMaxLevel() const;
   // text form of HashGridStorage::serialize (HashGridStorage.cpp:217-243),
-  // trivial bounding box, format version 9
+  // trivial bounding box, format version 9; unserialize reads versions 1-9
   std::string serialize() const;
   void unserialize(const std::string& text);
+  void recalcLeafProperty();  // HashGridStorage.cpp:358-397
 
  private:
   size_t dim_;
