@@ -49,6 +49,9 @@ def parse():
                     help="serve only; skip the one-launch-per-request FP64 baseline")
     ap.add_argument("--serve-autotune-cache", default=None,
                     help="file of tuned tile shapes, read and updated by both servers")
+    ap.add_argument("--serve-operator-sample", type=int, default=1000,
+                    help="requests timed with an operator built per request (the reference's "
+                         "predict pattern); 0 skips it")
     ap.add_argument("--solve", action="store_true",
                     help="run the regression-solve workload (FP64 CG vs mixed precision) instead")
     ap.add_argument("--inner-epsilon", type=float, default=1e-5)
@@ -205,6 +208,34 @@ def drive_server(srv, ids, c, warm=32):
                 stats=srv.stats()), outs
 
 
+def operator_per_request(c, n):
+    """The reference's prediction pattern (LearnerLeastSquaresIdentity::predict,
+    LearnerLeastSquaresIdentity.cpp:61-68): per request, an FP64 operator is
+    built on the query points, the model's grid is prepared and mult runs;
+    host arrays in and out. Closed loop over the first n requests of the
+    stream (no arrival times): evals/s = queries / wall time."""
+    from sgpp_amd import capi
+    dims = [m["dim"] for m in c["models"]]
+    lvix = [(m["level"], m["index"]) for m in c["models"]]
+    n = min(n, len(c["q"]))
+    t0 = time.perf_counter()
+    evals = 0
+    for k in range(n):
+        m, q, off = int(c["model"][k]), int(c["q"][k]), int(c["offset"][k])
+        X = c["pools"][dims[m]][off:off + q]
+        op = capi.MultiEval(c["models"][m]["basis"], X, precision=capi.PREC_FP64)
+        op.prepare(*lvix[m])
+        op.mult(c["models"][m]["alpha"])
+        del op
+        evals += q
+        if (k + 1) % 250 == 0:
+            print("[serve] operator per request: %d done, %.1f s" % (k + 1, time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    return {"requests": n, "queries": evals, "wall_s": wall, "evals_per_s": evals / wall,
+            "mean_ms_per_request": wall / n * 1e3}
+
+
 def serve_bench(args):
     import numpy as np
 
@@ -269,6 +300,13 @@ def serve_bench(args):
         "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
         "setup_s": gen_s,
     }
+    if args.serve_operator_sample > 0:
+        print("[serve] operator per request: %d requests" % args.serve_operator_sample,
+              file=sys.stderr, flush=True)
+        opr = operator_per_request(c, args.serve_operator_sample)
+        line["operator_per_request_fp64"] = opr
+        line["evals_ratio_vs_operator_per_request"] = (runs["served"]["evals_per_s"] /
+                                                       opr["evals_per_s"])
     print(json.dumps(line))
 
 
