@@ -359,6 +359,47 @@ sgpp_hip_status sgpp_hip_learner_performance(sgpp_hip_grid* grid, size_t num_ins
                                              size_t size_datatype, int reuse_alpha,
                                              double* gflop, double* gbyte);
 
+/* --------------------------------------------------------- fitter
+ * The least-squares model fitter of the datamining pipeline
+ * (ModelFittingLeastSquares, datadriven/src/sgpp/datadriven/datamining/
+ * modules/fitting/ModelFittingLeastSquares.cpp:30-142): fit (regular grid,
+ * solve), adapt (one surplus refinement, warm-started solve), update (new
+ * data on the current grid, warm-started solve; the reference's update frees
+ * its grid before using it, this one keeps it) and evaluate. Only CG is
+ * provided. The fitter owns copies of the data it was given. */
+typedef struct sgpp_hip_fitter sgpp_hip_fitter;
+
+/* FitterConfigurationLeastSquares (FitterConfiguration.cpp:150-220 defaults:
+ * 1 refinement of 1 point, CG eps 1e-12 / 100 iterations, lambda 0.01);
+ * grid.dim is ignored (taken from the data) */
+typedef struct sgpp_hip_fitter_config {
+  sgpp_hip_grid_config grid;
+  sgpp_hip_adaptivity_config refinement;
+  sgpp_hip_solver_config solver_refine;
+  sgpp_hip_solver_config solver_final;
+  double lambda;
+} sgpp_hip_fitter_config;
+
+sgpp_hip_status sgpp_hip_fitter_create(const sgpp_hip_fitter_config* cfg,
+                                       const sgpp_hip_params* params, sgpp_hip_fitter** out);
+sgpp_hip_status sgpp_hip_fitter_destroy(sgpp_hip_fitter* f);
+/* data: M x d row-major; targets: M */
+sgpp_hip_status sgpp_hip_fitter_fit(sgpp_hip_fitter* f, const double* data, size_t M, size_t d,
+                                    const double* targets);
+/* *refined = 1 when the grid grew and the model was solved again */
+sgpp_hip_status sgpp_hip_fitter_adapt(sgpp_hip_fitter* f, int* refined);
+sgpp_hip_status sgpp_hip_fitter_update(sgpp_hip_fitter* f, const double* data, size_t M,
+                                       size_t d, const double* targets);
+sgpp_hip_status sgpp_hip_fitter_evaluate(sgpp_hip_fitter* f, const double* data, size_t M,
+                                         size_t d, double* out);
+sgpp_hip_status sgpp_hip_fitter_grid_size(sgpp_hip_fitter* f, size_t* n);
+sgpp_hip_status sgpp_hip_fitter_level_index(sgpp_hip_fitter* f, uint32_t* level,
+                                            uint32_t* index);
+sgpp_hip_status sgpp_hip_fitter_alpha(sgpp_hip_fitter* f, double* alpha);
+/* CG iterations of the last solve and refinements since fit / update */
+sgpp_hip_status sgpp_hip_fitter_info(sgpp_hip_fitter* f, size_t* last_iterations,
+                                     size_t* refinements);
+
 #ifdef __cplusplus
 }
 #endif

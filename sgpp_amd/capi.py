@@ -102,6 +102,12 @@ class LearnerTiming(ctypes.Structure):
                 ("gbyte", ctypes.c_double)]
 
 
+class FitterConfig(ctypes.Structure):
+    _fields_ = [("grid", GridConfig), ("refinement", AdaptivityConfig),
+                ("solver_refine", SolverConfig), ("solver_final", SolverConfig),
+                ("lambda_", ctypes.c_double)]
+
+
 class LearnerStep(ctypes.Structure):
     _fields_ = [("grid_size", ctypes.c_size_t), ("iterations", ctypes.c_size_t),
                 ("refinements", ctypes.c_size_t), ("residual", ctypes.c_double),
@@ -189,6 +195,17 @@ SIGNATURES = {
     "sgpp_hip_learner_step_info": (_st, [_p, _sz, ctypes.POINTER(LearnerStep)]),
     "sgpp_hip_learner_performance": (_st, [_p, _sz, ctypes.c_int, _sz, _sz, ctypes.c_int, _dp,
                                            _dp]),
+    "sgpp_hip_fitter_create": (_st, [ctypes.POINTER(FitterConfig), ctypes.POINTER(Params),
+                                     ctypes.POINTER(_p)]),
+    "sgpp_hip_fitter_destroy": (_st, [_p]),
+    "sgpp_hip_fitter_fit": (_st, [_p, _dp, _sz, _sz, _dp]),
+    "sgpp_hip_fitter_adapt": (_st, [_p, ctypes.POINTER(ctypes.c_int)]),
+    "sgpp_hip_fitter_update": (_st, [_p, _dp, _sz, _sz, _dp]),
+    "sgpp_hip_fitter_evaluate": (_st, [_p, _dp, _sz, _sz, _dp]),
+    "sgpp_hip_fitter_grid_size": (_st, [_p, ctypes.POINTER(_sz)]),
+    "sgpp_hip_fitter_level_index": (_st, [_p, _up, _up]),
+    "sgpp_hip_fitter_alpha": (_st, [_p, _dp]),
+    "sgpp_hip_fitter_info": (_st, [_p, ctypes.POINTER(_sz), ctypes.POINTER(_sz)]),
 }
 
 F = {}
@@ -626,3 +643,76 @@ class Learner:
             check(F["sgpp_hip_learner_step_info"](self.h, k, ctypes.byref(st)))
             out.append({f: getattr(st, f) for f, _ in LearnerStep._fields_})
         return out
+
+
+class Fitter:
+    """ModelFittingLeastSquares on the GPU operator (sgpp_hip_fitter_*).
+
+    Defaults follow FitterConfiguration::setupDefaults: one refinement of one
+    point, CG to eps 1e-12 in at most 100 iterations, lambda 0.01."""
+
+    def __init__(self, basis=BASIS_LINEAR, level=3, num_refinements=1, num_points=1,
+                 threshold=0.0, refine=(1e-12, 100), final=(1e-12, 100), lam=0.01,
+                 precision=PREC_FP64, device=0):
+        cfg = FitterConfig(GridConfig(basis, 0, level),
+                           AdaptivityConfig(num_refinements, threshold, num_points, 0),
+                           SolverConfig(0, refine[0], refine[1], 0.0),
+                           SolverConfig(0, final[0], final[1], 0.0), lam)
+        prm = Params(precision, 0, device, 0, 0, 0)
+        h = _p()
+        check(F["sgpp_hip_fitter_create"](ctypes.byref(cfg), ctypes.byref(prm), ctypes.byref(h)))
+        self.h = h
+        self.d = 0
+
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h and F is not None:
+            F["sgpp_hip_fitter_destroy"](h)
+            self.h = None
+
+    def fit(self, X, y):
+        X, y = _f64(X), _f64(y)
+        check(F["sgpp_hip_fitter_fit"](self.h, _dptr(X), X.shape[0], X.shape[1], _dptr(y)))
+        self.d = X.shape[1]
+        return self
+
+    def adapt(self):
+        r = ctypes.c_int(0)
+        check(F["sgpp_hip_fitter_adapt"](self.h, ctypes.byref(r)))
+        return bool(r.value)
+
+    def update(self, X, y):
+        X, y = _f64(X), _f64(y)
+        check(F["sgpp_hip_fitter_update"](self.h, _dptr(X), X.shape[0], X.shape[1], _dptr(y)))
+        self.d = X.shape[1]
+        return self
+
+    def evaluate(self, X):
+        X = _f64(X)
+        out = np.empty(X.shape[0])
+        check(F["sgpp_hip_fitter_evaluate"](self.h, _dptr(X), X.shape[0], X.shape[1], _dptr(out)))
+        return out
+
+    @property
+    def grid_size(self):
+        n = _sz(0)
+        check(F["sgpp_hip_fitter_grid_size"](self.h, ctypes.byref(n)))
+        return n.value
+
+    def level_index(self):
+        n = self.grid_size
+        lv = np.empty((n, self.d), dtype=np.uint32)
+        ix = np.empty((n, self.d), dtype=np.uint32)
+        check(F["sgpp_hip_fitter_level_index"](self.h, lv.ctypes.data_as(_up),
+                                               ix.ctypes.data_as(_up)))
+        return lv, ix
+
+    def alpha(self):
+        a = np.empty(self.grid_size)
+        check(F["sgpp_hip_fitter_alpha"](self.h, _dptr(a)))
+        return a
+
+    def info(self):
+        it, ref = _sz(0), _sz(0)
+        check(F["sgpp_hip_fitter_info"](self.h, ctypes.byref(it), ctypes.byref(ref)))
+        return {"last_iterations": it.value, "refinements": ref.value}

@@ -12,6 +12,7 @@
 #include "meval_device.hpp"
 #include "sgpp/base/grid/Grid.hpp"
 #include "sgpp/datadriven/application/LearnerLeastSquaresIdentity.hpp"
+#include "sgpp/datadriven/datamining/ModelFittingLeastSquares.hpp"
 #include "sgpp/datadriven/operation/OperationMultiEvalHIP.hpp"
 #include "solver_device.hpp"
 
@@ -26,6 +27,10 @@ struct sgpp_hip_grid {
 
 struct sgpp_hip_learner {
   std::unique_ptr<sgpp::datadriven::LearnerLeastSquaresIdentity> learner;
+};
+
+struct sgpp_hip_fitter {
+  std::unique_ptr<sgpp::datadriven::ModelFittingLeastSquares> fitter;
 };
 
 struct sgpp_hip_meval {
@@ -660,6 +665,124 @@ sgpp_hip_status sgpp_hip_learner_step_info(sgpp_hip_learner* l, size_t k,
     out->residual = steps[k].residual;
     out->train_accuracy = steps[k].trainAccuracy;
     out->seconds = steps[k].seconds;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_create(const sgpp_hip_fitter_config* cfg,
+                                       const sgpp_hip_params* params, sgpp_hip_fitter** out) {
+  return guarded([&] {
+    REQUIRE(cfg && out, "null argument");
+    REQUIRE(cfg->refinement.error_based_refinement == 0, "error-based refinement is not provided");
+    sgpp::datadriven::FitterConfigurationLeastSquares c;
+    c.grid.type_ = basis_of(cfg->grid.basis);
+    c.grid.level_ = cfg->grid.level;
+    c.refinement.numRefinements_ = cfg->refinement.num_refinements;
+    c.refinement.refinementThreshold_ = cfg->refinement.refinement_threshold;
+    c.refinement.numRefinementPoints_ = cfg->refinement.num_refinement_points;
+    c.solverRefine = solver_config_of(&cfg->solver_refine);
+    c.solverFinal = solver_config_of(&cfg->solver_final);
+    c.lambda = cfg->lambda;
+    c.multipleEval = sgpp::datadriven::OperationMultipleEvalConfiguration(
+        sgpp::datadriven::OperationMultipleEvalType::STREAMING,
+        sgpp::datadriven::OperationMultipleEvalSubType::HIP, hip_config_of(params), "fitter");
+    auto f = std::unique_ptr<sgpp_hip_fitter>(new sgpp_hip_fitter());
+    f->fitter.reset(new sgpp::datadriven::ModelFittingLeastSquares(c));
+    *out = f.release();
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_destroy(sgpp_hip_fitter* f) {
+  return guarded([&] {
+    delete f;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_fit(sgpp_hip_fitter* f, const double* data, size_t M, size_t d,
+                                    const double* targets) {
+  return guarded([&] {
+    REQUIRE(f && data && targets, "null argument");
+    REQUIRE(M > 0 && d >= 1 && d <= 64, "bad data shape");
+    f->fitter->fit(DataMatrix(data, M, d), DataVector(targets, M));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_adapt(sgpp_hip_fitter* f, int* refined) {
+  return guarded([&] {
+    REQUIRE(f && refined, "null argument");
+    REQUIRE(f->fitter->isFitted(), "fitter has no model yet");
+    *refined = f->fitter->adapt() ? 1 : 0;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_update(sgpp_hip_fitter* f, const double* data, size_t M,
+                                       size_t d, const double* targets) {
+  return guarded([&] {
+    REQUIRE(f && data && targets, "null argument");
+    REQUIRE(M > 0 && d >= 1 && d <= 64, "bad data shape");
+    REQUIRE(!f->fitter->isFitted() || d == f->fitter->getGrid().getDimension(),
+            "data dimension differs from the grid's");
+    f->fitter->update(DataMatrix(data, M, d), DataVector(targets, M));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_evaluate(sgpp_hip_fitter* f, const double* data, size_t M,
+                                         size_t d, double* out) {
+  return guarded([&] {
+    REQUIRE(f && data && out, "null argument");
+    REQUIRE(f->fitter->isFitted(), "fitter has no model yet");
+    REQUIRE(M > 0 && d == f->fitter->getGrid().getDimension(),
+            "data dimension differs from the grid's");
+    DataMatrix X(data, M, d);
+    DataVector r;
+    f->fitter->evaluate(X, r);
+    std::memcpy(out, r.data(), M * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_grid_size(sgpp_hip_fitter* f, size_t* n) {
+  return guarded([&] {
+    REQUIRE(f && n, "null argument");
+    *n = f->fitter->isFitted() ? f->fitter->getGrid().getSize() : 0;
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_level_index(sgpp_hip_fitter* f, uint32_t* level,
+                                            uint32_t* index) {
+  return guarded([&] {
+    REQUIRE(f && level && index, "null argument");
+    REQUIRE(f->fitter->isFitted(), "fitter has no model yet");
+    std::vector<uint32_t> lv, ix;
+    f->fitter->getGrid().getStorage().getLevelIndexArrays(lv, ix);
+    std::memcpy(level, lv.data(), lv.size() * sizeof(uint32_t));
+    std::memcpy(index, ix.data(), ix.size() * sizeof(uint32_t));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_alpha(sgpp_hip_fitter* f, double* alpha) {
+  return guarded([&] {
+    REQUIRE(f && alpha, "null argument");
+    REQUIRE(f->fitter->isFitted(), "fitter has no model yet");
+    const DataVector& a = f->fitter->getSurpluses();
+    std::memcpy(alpha, a.data(), a.size() * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_fitter_info(sgpp_hip_fitter* f, size_t* last_iterations,
+                                     size_t* refinements) {
+  return guarded([&] {
+    REQUIRE(f && last_iterations && refinements, "null argument");
+    *last_iterations = f->fitter->getLastIterations();
+    *refinements = f->fitter->getRefinementsPerformed();
     return SGPP_HIP_OK;
   });
 }

@@ -1,5 +1,6 @@
 // Regression learner on the HIP operator (see LearnerLeastSquaresIdentity.hpp).
 #include "sgpp/datadriven/application/LearnerLeastSquaresIdentity.hpp"
+#include "sgpp/datadriven/datamining/ModelFittingLeastSquares.hpp"
 
 #include <chrono>
 #include <cstdint>
@@ -289,6 +290,122 @@ double LearnerLeastSquaresIdentity::getAccuracy(const DataVector& computed,
         (computed[i] < threshold && reference[i] < 0.0))
       ++correct;
   return static_cast<double>(correct) / static_cast<double>(computed.getSize());
+}
+
+// ------------------------------------------------ ModelFittingLeastSquares
+
+ModelFittingLeastSquares::ModelFittingLeastSquares(const FitterConfigurationLeastSquares& config)
+    : config_(config) {
+  if (config_.multipleEval.getSubType() == OperationMultipleEvalSubType::DEFAULT &&
+      config_.multipleEval.getType() == OperationMultipleEvalType::DEFAULT)
+    config_.multipleEval = OperationMultipleEvalConfiguration(
+        OperationMultipleEvalType::STREAMING, OperationMultipleEvalSubType::HIP,
+        HIPOperationConfiguration(), "fitter");
+  hip_parameters(config_.multipleEval);  // validates
+  if (config_.grid.type_ != base::GridType::Linear &&
+      config_.grid.type_ != base::GridType::ModLinear)
+    throw base::application_exception("ModelFittingLeastSquares: unsupported grid type");
+  if (config_.grid.level_ < 1)
+    throw base::application_exception("ModelFittingLeastSquares: grid level < 1");
+  if (config_.solverRefine.type_ != solver::SLESolverType::CG ||
+      config_.solverFinal.type_ != solver::SLESolverType::CG)
+    throw base::application_exception("ModelFittingLeastSquares: only CG is provided");
+}
+
+ModelFittingLeastSquares::~ModelFittingLeastSquares() {}
+
+base::Grid& ModelFittingLeastSquares::getGrid() {
+  if (!grid_) throw base::application_exception("ModelFittingLeastSquares: no grid yet");
+  return *grid_;
+}
+
+// ModelFittingLeastSquares.cpp:49-61
+void ModelFittingLeastSquares::fit(const DataMatrix& data, const DataVector& targets) {
+  if (data.getNrows() != targets.getSize() || data.getNrows() == 0)
+    throw base::application_exception("ModelFittingLeastSquares: data and targets differ in size");
+  grid_.reset();
+  refinementsPerformed_ = 0;
+  data_.reset(new DataMatrix(data));
+  targets_.reset(new DataVector(targets));
+  grid_.reset(new base::Grid(config_.grid.type_, data.getNcols()));
+  grid_->getGenerator().regular(static_cast<size_t>(config_.grid.level_));
+  alpha_ = DataVector(grid_->getSize(), 0.0);
+  assembleSystemAndSolve(config_.solverFinal);
+}
+
+// ModelFittingLeastSquares.cpp:63-100 (no interaction stencils)
+bool ModelFittingLeastSquares::adapt() {
+  if (!grid_)
+    throw base::application_exception(
+        "ModelFittingLeastSquares: Can't refine before initial grid is created");
+  if (refinementsPerformed_ >= config_.refinement.numRefinements_) return false;
+  base::SurplusRefinementFunctor f(alpha_, config_.refinement.numRefinementPoints_,
+                                   config_.refinement.refinementThreshold_);
+  const size_t before = grid_->getSize();
+  grid_->getGenerator().refine(f);
+  if (grid_->getSize() <= before) return false;
+  alpha_.resize(grid_->getSize(), 0.0);
+  assembleSystemAndSolve(config_.solverRefine);
+  refinementsPerformed_++;
+  return true;
+}
+
+// ModelFittingLeastSquares.cpp:102-111; keeps the grid (see the header)
+void ModelFittingLeastSquares::update(const DataMatrix& data, const DataVector& targets) {
+  if (!grid_) {
+    fit(data, targets);
+    return;
+  }
+  if (data.getNrows() != targets.getSize() || data.getNrows() == 0)
+    throw base::application_exception("ModelFittingLeastSquares: data and targets differ in size");
+  if (data.getNcols() != grid_->getDimension())
+    throw base::application_exception("ModelFittingLeastSquares: data dimension differs");
+  refinementsPerformed_ = 0;
+  data_.reset(new DataMatrix(data));
+  targets_.reset(new DataVector(targets));
+  assembleSystemAndSolve(config_.solverFinal);
+}
+
+// ModelFittingLeastSquares.cpp:37-40, through the operator on one point
+double ModelFittingLeastSquares::evaluate(const DataVector& sample) {
+  DataMatrix one(sample.data(), 1, sample.getSize());
+  DataVector r(1);
+  evaluate(one, r);
+  return r[0];
+}
+
+// ModelFittingLeastSquares.cpp:43-47
+void ModelFittingLeastSquares::evaluate(DataMatrix& samples, DataVector& results) {
+  if (!grid_) throw base::application_exception("ModelFittingLeastSquares: not fitted");
+  if (samples.getNcols() != grid_->getDimension())
+    throw base::application_exception("ModelFittingLeastSquares: sample dimension differs");
+  results.resize(samples.getNrows());
+  std::unique_ptr<base::OperationMultipleEval> op(
+      op_factory::createOperationMultipleEval(*grid_, samples, config_.multipleEval));
+  op->mult(alpha_, results);
+}
+
+// ModelFittingLeastSquares.cpp:129-140: system on the current grid and data,
+// b = B^T y, CG from the current alpha (reuse = true)
+void ModelFittingLeastSquares::assembleSystemAndSolve(const solver::SLESolverConfiguration& sc) {
+  HIPOperationConfiguration hp = hip_parameters(config_.multipleEval);
+  const bool mixed = hp.precision == HIPPrecision::FP32_FP64ACC;
+  if (mixed) hp.keepFP64 = true;
+  OperationMultipleEvalConfiguration cfg(OperationMultipleEvalType::STREAMING,
+                                         OperationMultipleEvalSubType::HIP, hp, "fitter");
+  std::unique_ptr<base::OperationMultipleEval> B(
+      op_factory::createOperationMultipleEval(*grid_, *data_, cfg));
+  auto* hop = dynamic_cast<OperationMultiEvalHIP*>(B.get());
+  if (!hop) throw base::factory_exception("ModelFittingLeastSquares: not a HIP operator");
+  sgpp_hip::DeviceRegression system(hop->device());
+  sgpp_hip::SolveOptions o;
+  o.mixed = mixed;
+  o.lambda = config_.lambda;
+  o.max_iterations = sc.maxIterations_;
+  o.epsilon = sc.eps_;
+  o.reuse = true;
+  const sgpp_hip::SolveResult r = system.solve(targets_->getPointer(), alpha_.getPointer(), o);
+  lastIterations_ = r.iterations;
 }
 
 }  // namespace datadriven

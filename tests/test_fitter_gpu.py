@@ -1,0 +1,99 @@
+"""ModelFittingLeastSquares (fit / adapt / update / evaluate) on the GPU
+operator against reference parts.
+
+fit + k adapt calls take the steps of oracle.learner_train with k
+refinements when the refinement and final solver configurations agree:
+regular grid, CG from alpha = 0, then per step a surplus refinement, alpha
+zero-extended and CG from it (reuse = true). oracle.learner_train runs those
+steps on the reference's Grid, StreamingDefault operator, SurplusRefinement-
+Functor and ConjugateGradients.
+
+Tolerances: grids identical point for point; alpha within 1e-8 relative
+(CG to 1e-12); evaluate within 1e-12 of the definition of B alpha; update
+(a warm start from another data set's solution) within 1e-6 of a fresh fit on
+the new data; FP32 products with FP64 refinement within 1e-6 of FP64."""
+import numpy as np
+import pytest
+
+import oracle
+from sgpp_amd import workloads
+
+pytestmark = pytest.mark.gpu
+ref_only = pytest.mark.skipif(oracle.ref() is None, reason="reference build not available")
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def dataset(seed=41, M=3000, d=3):
+    X = workloads.uniform_points(seed, M, d)
+    y = np.sin(2 * np.pi * X[:, 0]) * X[:, 1] + X[:, 2] ** 2
+    y += 0.01 * workloads.normal(workloads.stream(seed, 5), M)
+    return X, y
+
+
+@ref_only
+@pytest.mark.parametrize("basis", [0, 1])
+def test_fit_and_adapt_match_reference_steps(gpu, basis):
+    capi = gpu
+    X, y = dataset()
+    cg = (1e-12, 1000)
+    rg, ra, rsteps = oracle.learner_train(basis, X, y, 3, 1e-4, refine=cg, num_refinements=3,
+                                          num_points=5)
+    f = capi.Fitter(basis, level=3, num_refinements=3, num_points=5, refine=cg, final=cg,
+                    lam=1e-4)
+    f.fit(X, y)
+    assert f.grid_size == rsteps[0][0]
+    sizes = [f.grid_size]
+    while f.adapt():
+        sizes.append(f.grid_size)
+    assert sizes == [n for n, _, _ in rsteps]
+    assert f.info()["refinements"] == 3
+    assert not f.adapt()  # the configured number of refinements is done
+    lv, ix = f.level_index()
+    rlv, rix = rg.level_index()
+    assert np.array_equal(lv, rlv) and np.array_equal(ix, rix)
+    assert rel(f.alpha(), ra) <= 1e-8
+    Xt, _ = dataset(seed=42, M=500)
+    assert rel(f.evaluate(Xt), oracle.mult(basis, Xt, lv, ix, f.alpha())) <= 1e-12
+
+
+def test_update_warm_starts_on_new_data(gpu):
+    capi = gpu
+    XA, yA = dataset(seed=43)
+    XB, yB = dataset(seed=44)
+    cg = (1e-12, 1000)
+    f = capi.Fitter(0, level=4, num_refinements=0, refine=cg, final=cg, lam=1e-4).fit(XA, yA)
+    alpha_a = f.alpha()
+    f.update(XB, yB)
+    g = capi.Fitter(0, level=4, num_refinements=0, refine=cg, final=cg, lam=1e-4).fit(XB, yB)
+    assert rel(f.alpha(), g.alpha()) <= 1e-6
+    assert rel(f.alpha(), alpha_a) > 1e-3  # the new data changed the model
+
+
+def test_mixed_precision_fitter(gpu):
+    capi = gpu
+    X, y = dataset(seed=45)
+    cg = (1e-10, 1000)
+    f64 = capi.Fitter(1, level=3, num_refinements=2, num_points=3, refine=cg, final=cg,
+                      lam=1e-4).fit(X, y)
+    f32 = capi.Fitter(1, level=3, num_refinements=2, num_points=3, refine=cg, final=cg,
+                      lam=1e-4, precision=capi.PREC_FP32).fit(X, y)
+    while f64.adapt():
+        assert f32.adapt()
+    assert f64.grid_size == f32.grid_size
+    assert rel(f32.alpha(), f64.alpha()) <= 1e-6
+
+
+def test_fitter_defaults_and_errors(gpu):
+    capi = gpu
+    f = capi.Fitter()  # FitterConfiguration defaults: level 3, lambda 0.01
+    with pytest.raises(capi.SGppHIPError):
+        f.adapt()  # no model yet
+    X, y = dataset(seed=46, M=500)
+    f.fit(X, y)
+    assert f.grid_size == oracle.regular_size_bound(3, 3)
+    assert f.adapt() and not f.adapt()  # one refinement by default
+    with pytest.raises(capi.SGppHIPError):
+        f.evaluate(np.zeros((4, 2)))  # wrong dimension
