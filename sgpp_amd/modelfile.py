@@ -15,6 +15,12 @@ the operator:
 * `read_alpha` reads an alpha file: an ARFF file with one column and no
   targets (the reference's `LearnedKnowledgeFormatter` writes them with the
   attribute `alpha`).
+* `write_alpha` writes one in the layout of the reference's
+  `LearnedKnowledgeFormatter.__writeAlphaARFF`
+  (datadriven/python/learner/formatter/LearnedKnowledgeFormatter.py:97-105):
+  relation line, the attribute `alpha`, values as "%1.20f" (fixed point, so
+  magnitudes below 1e-20 are written as zero); `save_model` writes the grid
+  (format version 9) and its alpha file.
 * `load_model` reads a serialized grid (`Grid::unserialize`, format versions
   1 to 9; `capi.Grid.unserialize`) and its alpha file, and checks that they
   agree in size.
@@ -113,3 +119,25 @@ def load_model(grid_source, alpha_source):
         raise ARFFError("alpha file holds %d coefficients, the grid %d points"
                         % (alpha.shape[0], grid.size))
     return grid, alpha
+
+
+def write_alpha(path, alpha, relation=None):
+    """Alpha ARFF file as the reference's formatter writes it."""
+    alpha = np.asarray(alpha, dtype=np.float64).ravel()
+    with open(path, "w") as f:
+        f.write("@RELATION \"%s ALPHAFILE\"\n\n" % (relation if relation is not None else path))
+        f.write("@ATTRIBUTE alpha NUMERIC\n")
+        f.write("\n@DATA\n")
+        for a in alpha:
+            f.write("%1.20f\n" % a)
+
+
+def save_model(grid_path, alpha_path, grid, alpha):
+    """Serialized grid (format version 9) and its alpha file."""
+    alpha = np.asarray(alpha, dtype=np.float64).ravel()
+    if alpha.shape[0] != grid.size:
+        raise ARFFError("alpha holds %d coefficients, the grid %d points"
+                        % (alpha.shape[0], grid.size))
+    with open(grid_path, "w") as f:
+        f.write(grid.serialize())
+    write_alpha(alpha_path, alpha)

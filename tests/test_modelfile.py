@@ -138,3 +138,30 @@ def test_alpha_file_must_match_the_grid():
         short = f.read().rsplit("\n", 2)[0] + "\n"
     with pytest.raises(modelfile.ARFFError):
         modelfile.load_model(GRID, short)
+
+
+def test_alpha_writer_reproduces_the_reference_file(tmp_path):
+    _, alpha = modelfile.load_model(GRID, ALPHA)
+    out = str(tmp_path / "a.arff")
+    modelfile.write_alpha(out, alpha, relation="classifier_test.a0.alpha.arff")
+    with open(ALPHA) as f:
+        want = f.read()
+    with open(out) as f:
+        got = f.read()
+    # the reference names the relation after its output path
+    assert got.split("\n")[1:] == want.split("\n")[1:len(got.split("\n"))]
+    assert modelfile.read_alpha(out).tolist() == alpha.tolist()
+
+
+def test_save_and_load_model(tmp_path):
+    g = capi.Grid.modlinear(4).regular(3)
+    g.refine_surplus(np.linspace(1.0, 2.0, g.size), 4)
+    alpha = np.linspace(-1.0, 1.0, g.size)
+    gp, ap = str(tmp_path / "m.grid"), str(tmp_path / "m.alpha.arff")
+    modelfile.save_model(gp, ap, g, alpha)
+    h, a = modelfile.load_model(gp, ap)
+    assert h.basis == capi.BASIS_MODLINEAR
+    for x, y in zip(g.level_index(), h.level_index()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(g.leaf(), h.leaf())
+    assert np.abs(a - alpha).max() <= 1e-20
