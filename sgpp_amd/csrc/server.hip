@@ -186,15 +186,22 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
 #pragma unroll
         for (int q = 0; q < J; ++q) p[q][k] = T(1);
       }
+      // shared dimensions first, in a loop of their own (skipped when the
+      // sub-tile shares none), then the per-point ones: two simple loops
+      // compile to fewer scalar instructions than one with both cases
+      const uint32_t msh = mask & shared, mrest = mask & ~shared;
+      if (msh != 0u) {
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (!((mask >> d) & 1u)) continue;
-        if ((shared >> d) & 1u) {  // one factor per query for the whole sub-tile
+        for (int d = 0; d < D; ++d) {
+          if (!((msh >> d) & 1u)) continue;
           const T L = sLI[(d * kTG + j) * 2], I = sLI[(d * kTG + j) * 2 + 1];
 #pragma unroll
           for (int k = 0; k < K; ++k) g[k] *= clamp01(T(1) - absT(fmaT(L, x[k][d], -I)));
-          continue;
         }
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        if (!((mrest >> d) & 1u)) continue;
         T li[2 * J];
 #pragma unroll
         for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
