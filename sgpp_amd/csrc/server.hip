@@ -82,13 +82,17 @@ template <typename T>
 constexpr int serve_k(int D) {
   return sizeof(T) == 8 ? (D <= 5 ? 4 : (D <= 10 ? 2 : 1)) : (D <= 4 ? 8 : (D <= 10 ? 4 : 2));
 }
-// grid points per sub-tile (one wave-uniform branch per sub-tile and
-// dimension); J*K running products per lane, capped so that every dimension
-// case of the persistent kernel stays within the same register budget
+// grid points per sub-tile (wave-uniform branches per sub-tile and
+// dimension): J*K = 32 (FP32) or 16 (FP64) running products per lane, J at
+// most 16 (FP32) or 8 (FP64) so that every dimension case of the persistent
+// kernel stays within one register budget (FP64 with J = 16 needs 180
+// registers). Larger sub-tiles spread a shared dimension's per-query factor
+// over more points.
 template <typename T>
 constexpr int serve_j(int D) {
-  return (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D) < 8 ? (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D)
-                                                          : 8;
+  return (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D) < (sizeof(T) == 8 ? 8 : 16)
+             ? (sizeof(T) == 8 ? 16 : 32) / serve_k<T>(D)
+             : (sizeof(T) == 8 ? 8 : 16);
 }
 
 // LDS staging buffer of one 64-point grid tile: weights A (kTG values),
