@@ -230,3 +230,27 @@ def test_failures_are_per_request(gpu):
     with pytest.raises(capi.SGppHIPError):
         srv.submit(ids[0], np.full((3, d0), 1.5))
     srv.close()
+
+
+@pytest.mark.parametrize("prec,tol", [(1, 1e-5), (0, 1e-12)])
+def test_shared_subtile_dimensions(gpu, prec, tol):
+    """Adaptive models whose sorted points share some dimensions within a
+    sub-tile and not others, at dimensions that give every sub-tile size
+    (FP32: 4 points at d <= 4, 8 at d <= 10, 16 above; FP64: 4 and 8)."""
+    capi = gpu
+    srv = small_server(capi, prec)
+    for i, (basis, d, level) in enumerate([(0, 3, 4), (1, 3, 4), (0, 8, 3), (1, 8, 3),
+                                            (0, 14, 2), (1, 14, 3)]):
+        g = capi.Grid(basis, d).regular(level)
+        g.refine_surplus(np.linspace(1.0, 3.0, g.size), 7)
+        lv, ix = g.level_index()
+        alpha = workloads.normal(workloads.stream(120 + i, 0), g.size)
+        mid = srv.admit_model(basis, lv, ix, alpha)
+        X = workloads.uniform_points(130 + i, 3000, d)
+        if prec == 1:
+            X = X.astype(np.float32).astype(np.float64)
+        rid, n = srv.submit(mid, X)
+        st, r, _ = srv.poll(rid, n)
+        assert st == capi.REQ_DONE
+        assert rel(r, oracle.mult(basis, X, lv, ix, alpha)) <= tol, (basis, d)
+    srv.close()
