@@ -73,6 +73,10 @@ constexpr size_t kDataSlack = 1024;
 
 // Points per lane (K) and grid points evaluated together (J). Values from
 // variant sweeps on the d=10 ModLinear, d=8 Linear and d=20 Linear workloads.
+// Kernels for d > 10 are held to 128 registers (4 waves per SIMD) although
+// some values then spill: at d = 20 that was 4-21 % faster than 3 waves
+// without spills; at d = 10 (FP64) it was 10-15 % slower, so d <= 10 keeps
+// its natural allocation.
 template <typename T>
 constexpr int mult_k(int D) {
   return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
@@ -111,7 +115,8 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * 
 // data points (each wave 64*K consecutive ones) x grid chunk [y*chunk,
 // +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
 template <int D, int K, int J, typename T, bool kSkip>
-__global__ __launch_bounds__(kBlock) void mult_kernel(const T* __restrict__ X, int64_t ldx,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
+void mult_kernel(const T* __restrict__ X, int64_t ldx,
                                                       const T* __restrict__ LI,
                                                       const float* __restrict__ TB,
                                                       const T* __restrict__ A,
@@ -199,7 +204,8 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // grid chunk [blockIdx.y*kChunkT, +kChunkT), the segment split into four
 // contiguous wave shares. part: [S][ldp].
 template <int D, int K, int J, typename T, bool kSkip>
-__global__ __launch_bounds__(kBlock) void multT_kernel(const T* __restrict__ X, int64_t ldx,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
+void multT_kernel(const T* __restrict__ X, int64_t ldx,
                                                        const T* __restrict__ Y,
                                                        const T* __restrict__ LI,
                                                        const float* __restrict__ TB,
