@@ -42,6 +42,14 @@ class ARFFError(ValueError):
     """Malformed ARFF content (the reference's file_exception)."""
 
 
+def _text_of(source):
+    """The source itself when it holds a newline, else the named file's text."""
+    if "\n" in source:
+        return source
+    with open(source) as f:
+        return f.read()
+
+
 def _data_lines(text):
     for line in text.split("\n"):
         line = line.rstrip("\r")
@@ -55,8 +63,7 @@ def read_arff(source, has_targets=True, instance_cutoff=None, selected_cols=(),
     """Returns (data [n x dim] float64, targets [n] float64 or None).
 
     source: a path or the file's text (text when it contains a newline)."""
-    text = source if "\n" in source else open(source).read()
-    lines = list(_data_lines(text))
+    lines = list(_data_lines(_text_of(source)))
     dim = 0
     n_inst = 0
     for k, line in enumerate(lines):
@@ -112,8 +119,7 @@ def load_model(grid_source, alpha_source):
 
     grid_source / alpha_source: paths or the files' text."""
     from . import capi
-    text = grid_source if "\n" in grid_source else open(grid_source).read()
-    grid = capi.Grid.unserialize(text)
+    grid = capi.Grid.unserialize(_text_of(grid_source))
     alpha = read_alpha(alpha_source)
     if alpha.shape[0] != grid.size:
         raise ARFFError("alpha file holds %d coefficients, the grid %d points"
