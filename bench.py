@@ -11,11 +11,19 @@ over all M x N pairs, with inputs resident in HBM.
 `value` is the FP32-with-FP64-accumulation form's pair-evals/s (2*M*N per
 step); the plain FP64 kernel is timed beside it ("fp64", "speedup_vs_fp64").
 Both forms' relative errors against the oracle are reported on a sample.
+roofline: the dominant FP32 kernel's executed flops (counted in the run by
+the counting instantiation of the same kernel) over its HIP-event time,
+against the FP32 vector peak; the 6d-per-pair figure over all M*N pairs is
+reported beside it as achieved_6d_convention.
 cpu_baseline: the reference's StreamingDefault ModLinear operator
 (OperationMultiEvalModMaskStreaming, compiled from the reference sources into
-oracle/_ref) on the host cores, FP64, on a bounded data sample.
+oracle/_ref) on the host cores, FP64, on a bounded data sample (warm-up, then
+the median of 5).
+served: the served-loop workload (64 resident models, Poisson requests) for
+a short stream at 2000 and at 8000 req/s, the FP32 continuous-batching server
+against one launch per request in plain FP64: evals/s, p50/p99 latency.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-serve]
 """
 import argparse
 import json
@@ -62,6 +70,10 @@ def parse():
     ap.add_argument("--copy-reference-mib", type=int, default=0,
                     help="first copy this many MiB device to device three times (a kernel "
                          "of known traffic for scaling hardware byte counters)")
+    ap.add_argument("--no-serve", action="store_true",
+                    help="skip the served-loop leg of the default run")
+    ap.add_argument("--serve-leg-seconds", type=float, default=1.5,
+                    help="seconds of each request stream (2000 and 8000 req/s) in the default run")
     ap.add_argument("--grid-target", type=int, default=100000)
     ap.add_argument("--data-points", type=int, default=1000000)
     return ap.parse_args()
@@ -108,7 +120,22 @@ def accuracy(basis, X, lv, ix, alpha, y, results):
     return out
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(basis, X, grid, lv, ix, alpha, y, seconds):
+    """The reference's StreamingDefault operator (or the C restatement when the
+    reference build is absent) on the host cores: one warm-up run, then the
+    median of 5 timed mult+multTranspose runs on a data sample sized to about
+    `seconds` per run."""
     import numpy as np
 
     import oracle
@@ -118,33 +145,36 @@ def cpu_baseline(basis, X, grid, lv, ix, alpha, y, seconds):
     # the reference operator reads the same points, inserted in the same order
     rgrid = oracle.RefGrid.from_arrays(basis, lv, ix) if kind == "reference" else None
 
-    def run(m):
+    def make(m):
         if kind == "reference":
-            op = oracle.RefOp(rgrid, oracle.KIND_STREAMING, X[:m])
-            op.mult(alpha)  # warm-up, first-touch
-            t = time.perf_counter()
-            op.mult(alpha)
-            op.mult_transpose(y[:m])
-            return time.perf_counter() - t
-        op = oracle.Port(basis, X[:m], lv, ix)
-        op.mult(alpha)
+            return oracle.RefOp(rgrid, oracle.KIND_STREAMING, X[:m])
+        return oracle.Port(basis, X[:m], lv, ix)
+
+    def run(op, m):
         t = time.perf_counter()
         op.mult(alpha)
         op.mult_transpose(y[:m])
         return time.perf_counter() - t
 
     m0 = 20000
-    t0 = run(m0)
+    op = make(m0)
+    run(op, m0)
+    t0 = run(op, m0)
     m = int(min(X.shape[0], max(m0, m0 * seconds / max(t0, 1e-3))))
-    times = sorted(run(m) for _ in range(3))
-    med = times[1]
+    op = make(m)
+    run(op, m)  # warm-up
+    times = sorted(run(op, m) for _ in range(5))
+    med = times[2]
     return {
         "value": 2.0 * m * N / med,
         "unit": "pair-evals/s",
         "cores": threads,
         "kind": kind,
+        "cpu": cpu_model(),
+        "run_seconds": times,
         "sample": ("%s StreamingDefault ModLinear operator (OperationMultiEvalModMaskStreaming%s), "
-                   "FP64, full %d-point grid x first %d data points, mult+multTranspose, median of 3"
+                   "FP64, full %d-point grid x first %d data points, mult+multTranspose, one "
+                   "warm-up then the median of 5"
                    % ("reference" if kind == "reference" else "restated",
                       ", %s build" % oracle.ref_variant() if kind == "reference" else "", N, m)),
     }
@@ -236,76 +266,126 @@ def operator_per_request(c, n):
             "mean_ms_per_request": wall / n * 1e3}
 
 
-def serve_bench(args):
+def thin_stream(c, frac, seed=12):
+    """Poisson thinning: each request of the stream kept with probability frac
+    (a seeded draw), which leaves a Poisson stream at frac times the rate."""
     import numpy as np
 
+    from sgpp_amd import workloads
+    keep = workloads.uniform(workloads.stream(5, seed), len(c["arrival"])) < frac
+    out = dict(c)
+    for k in ("arrival", "model", "q", "offset"):
+        out[k] = np.asarray(c[k])[keep]
+    out["rate"] = c["rate"] * frac
+    return out
+
+
+def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
+    """The served workload (64 resident models, Poisson requests) at each rate
+    of `rates` for `seconds`: one stream is drawn at the highest rate and the
+    lower rates are thinned from it. The FP32 server (continuous batching)
+    and the baseline (one launch per request, plain FP64, same resident
+    models) each admit the models once and replay every stream in real time.
+    Returns (per-rate results, the stream at the highest rate, outputs)."""
     from sgpp_amd import capi, workloads
+    rates = sorted(rates)
     t = time.perf_counter()
-    c = workloads.config_serving(rate=args.serve_rate, duration=args.serve_seconds)
+    c_hi = workloads.config_serving(rate=rates[-1], duration=seconds)
     gen_s = time.perf_counter() - t
-    N = np.array([m["level"].shape[0] for m in c["models"]])
-    D = np.array([m["dim"] for m in c["models"]])
-    flops = float(sum(6 * D[m] * N[m] * q for m, q in zip(c["model"], c["q"])))
-    runs = {}
-    outputs = {}
+    streams = {r: (c_hi if r == rates[-1] else thin_stream(c_hi, r / rates[-1])) for r in rates}
     configs = [("served", dict(precision=capi.PREC_FP32, max_inflight=2))]
-    if not args.serve_no_baseline:
+    if baseline:
         configs.append(("one_launch_per_request_fp64",
                         dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
                              use_graph=False)))
+    runs = {r: {} for r in rates}
+    outputs = {r: {} for r in rates}
     for name, kw in configs:
-        kw["autotune"] = not args.no_autotune
-        kw["autotune_cache"] = args.serve_autotune_cache
+        kw["autotune"] = autotune
+        kw["autotune_cache"] = cache
         t = time.perf_counter()
         srv = capi.Server(**kw)
         ids = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
-               for m in c["models"]]
+               for m in c_hi["models"]]
         admit_s = time.perf_counter() - t
         print("[serve] %s: %d models admitted in %.1f s" % (name, len(ids), admit_s),
               file=sys.stderr, flush=True)
-        res, outs = drive_server(srv, ids, c)
-        res["admit_and_tune_s"] = admit_s
-        res["tuned_shapes"] = {"%s_d%d" % ("modlinear" if b else "linear", d): srv.tuned_shape(b, d)
-                               for b, d in sorted({(m["basis"], m["dim"]) for m in c["models"]})
-                               if srv.tuned_shape(b, d) is not None}
-        res["achieved_tflops_6d"] = flops * res["requests_done"] / len(c["q"]) / res["span_s"] / 1e12
-        runs[name] = res
-        outputs[name] = outs
+        for r in rates:
+            res, outs = drive_server(srv, ids, streams[r])
+            res["admit_and_tune_s"] = admit_s
+            runs[r][name] = res
+            outputs[r][name] = outs
+            print("[serve] %s at %.0f req/s: %.3g evals/s, p99 %.1f ms" %
+                  (name, r, res["evals_per_s"], res["p99_ms"]), file=sys.stderr, flush=True)
+        if name == "served":
+            runs["tuned_shapes"] = {
+                "%s_d%d" % ("modlinear" if b else "linear", d): srv.tuned_shape(b, d)
+                for b, d in sorted({(m["basis"], m["dim"]) for m in c_hi["models"]})
+                if srv.tuned_shape(b, d) is not None}
         srv.close()
-    ref = outputs.get("one_launch_per_request_fp64", {})
-    errs = [np.linalg.norm(outputs["served"][k] - ref[k]) / max(np.linalg.norm(ref[k]), 1e-300)
-            for k in outputs["served"] if k in ref]
-    base = runs.get("one_launch_per_request_fp64")
+    import numpy as np
+    N = np.array([m["level"].shape[0] for m in c_hi["models"]])
+    D = np.array([m["dim"] for m in c_hi["models"]])
+    out = {"generation_s": gen_s, "seconds": seconds, "tuned_shapes": runs.pop("tuned_shapes", None),
+           "rates": {}}
+    for r in rates:
+        c = streams[r]
+        flops = float(sum(6 * D[m] * N[m] * q for m, q in zip(c["model"], c["q"])))
+        sv = runs[r]["served"]
+        sv["achieved_tflops_6d"] = (flops * sv["requests_done"] / len(c["q"]) / sv["span_s"]
+                                    / 1e12)
+        base = runs[r].get("one_launch_per_request_fp64")
+        ref = outputs[r].get("one_launch_per_request_fp64", {})
+        errs = [np.linalg.norm(outputs[r]["served"][k] - ref[k]) /
+                max(np.linalg.norm(ref[k]), 1e-300) for k in outputs[r]["served"] if k in ref]
+        out["rates"]["%d" % int(r)] = {
+            "requests": len(c["q"]), "mean_queries": float(c["q"].mean()),
+            "offered_evals_per_s": float(c["q"].sum() / seconds),
+            "offered_tflops_6d": flops / seconds / 1e12,
+            "served": sv, "baseline": base,
+            "evals_ratio_vs_baseline": sv["evals_per_s"] / base["evals_per_s"] if base else None,
+            "p99_ratio_vs_baseline": sv["p99_ms"] / base["p99_ms"] if base else None,
+            "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
+        }
+    return out, c_hi
+
+
+def serve_bench(args):
+    rates = [args.serve_rate]
+    leg, c = serve_leg(rates, args.serve_seconds, baseline=not args.serve_no_baseline,
+                       autotune=not args.no_autotune, cache=args.serve_autotune_cache)
+    r = leg["rates"]["%d" % int(args.serve_rate)]
+    N = [m["level"].shape[0] for m in c["models"]]
     line = {
         "metric": "served evals/s + p50/p99 latency",
-        "value": runs["served"]["evals_per_s"],
+        "value": r["served"]["evals_per_s"],
         "unit": "evals/s",
         "n_gpus": 1,
         "higher_is_better": True,
-        "p50_ms": runs["served"]["p50_ms"],
-        "p99_ms": runs["served"]["p99_ms"],
+        "p50_ms": r["served"]["p50_ms"],
+        "p99_ms": r["served"]["p99_ms"],
         "dtype": "fp32 products, fp64 accumulation",
-        "data": "synthetic, seeded: 64 models, Poisson %.0f req/s for %.1f s" % (c["rate"],
-                                                                              c["duration"]),
-        "config": {"workload": "served_64_models_poisson_%dreqs" % int(c["rate"]),
-                   "models": len(c["models"]), "grid_points_total": int(N.sum()),
-                   "requests": len(c["q"]), "mean_queries": float(c["q"].mean()),
-                   "offered_evals_per_s": float(c["q"].sum() / c["duration"]),
-                   "offered_tflops_6d": flops / c["duration"] / 1e12},
-        "served": runs["served"],
-        "baseline": base,
-        "evals_ratio_vs_baseline": runs["served"]["evals_per_s"] / base["evals_per_s"]
-        if base else None,
-        "p99_ratio_vs_baseline": runs["served"]["p99_ms"] / base["p99_ms"] if base else None,
-        "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
-        "setup_s": gen_s,
+        "data": "synthetic, seeded: 64 models, Poisson %.0f req/s for %.1f s" % (args.serve_rate,
+                                                                              args.serve_seconds),
+        "config": {"workload": "served_64_models_poisson_%dreqs" % int(args.serve_rate),
+                   "models": len(c["models"]), "grid_points_total": int(sum(N)),
+                   "requests": r["requests"], "mean_queries": r["mean_queries"],
+                   "offered_evals_per_s": r["offered_evals_per_s"],
+                   "offered_tflops_6d": r["offered_tflops_6d"]},
+        "served": r["served"],
+        "baseline": r["baseline"],
+        "evals_ratio_vs_baseline": r["evals_ratio_vs_baseline"],
+        "p99_ratio_vs_baseline": r["p99_ratio_vs_baseline"],
+        "max_rel_err_vs_fp64": r["max_rel_err_vs_fp64"],
+        "tuned_shapes": leg["tuned_shapes"],
+        "setup_s": leg["generation_s"],
     }
     if args.serve_operator_sample > 0:
         print("[serve] operator per request: %d requests" % args.serve_operator_sample,
               file=sys.stderr, flush=True)
         opr = operator_per_request(c, args.serve_operator_sample)
         line["operator_per_request_fp64"] = opr
-        line["evals_ratio_vs_operator_per_request"] = (runs["served"]["evals_per_s"] /
+        line["evals_ratio_vs_operator_per_request"] = (r["served"]["evals_per_s"] /
                                                        opr["evals_per_s"])
     print(json.dumps(line))
 
@@ -511,29 +591,35 @@ def main():
         res[name] = dict(step_s=step_s, mult_ms=mult_ms, multT_ms=multT_ms,
                          shape=op.last_shape())
         outs[name] = (d_r.cpu().numpy().copy(), d_a.cpu().numpy().copy())
+    # executed work of each kernel (counting instantiations, untimed)
+    work = {name: op.count_work(prec) for name, prec in (("fp64", capi.PREC_FP64),
+                                                          ("fp32", capi.PREC_FP32))}
 
     pairs = float(M) * float(N)
-    flops_per_launch = 6.0 * d * pairs
+    flops_6d = 6.0 * d * pairs
     v32 = 2.0 * pairs / res["fp32"]["step_s"]
     v64 = 2.0 * pairs / res["fp64"]["step_s"]
-    k32 = res["fp32"]
-    dom_ms = max(k32["mult_ms"], k32["multT_ms"])
-    dom = "mult_kernel" if k32["mult_ms"] >= k32["multT_ms"] else "multT_kernel"
-    achieved = flops_per_launch / (dom_ms * 1e-3) / 1e12
-    # HBM bytes and executed VALU instructions of the dominant kernel, from the
-    # committed rocprofv3 counter passes over this workload
-    traffic = None
-    executed = None
-    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % dom)
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            tj = json.load(f)
-        traffic = tj.get("bytes_per_launch")
-        executed = {k: tj.get(k) for k in ("valu_instr_per_pair_lane", "valu_instr_per_launch",
-                                           "salu_instr_per_launch",
-                                           "cycles_per_valu_instr_per_simd", "shader_clock_ghz",
-                                           "algorithmic_bytes_per_launch")}
 
+    def roof(name, peak):
+        k = res[name]
+        dom = "mult_kernel" if k["mult_ms"] >= k["multT_ms"] else "multT_kernel"
+        key = "mult" if dom == "mult_kernel" else "multT"
+        dom_ms = k["%s_ms" % key]
+        w = work[name]
+        executed = 4.0 * w["%s_hat_steps" % key] + 2.0 * w["%s_pairs" % key]
+        return dom, dom_ms, {
+            "achieved": executed / (dom_ms * 1e-3) / 1e12, "peak": peak,
+            "frac": executed / (dom_ms * 1e-3) / 1e12 / peak,
+            "executed_flops_per_launch": executed,
+            "hat_steps_per_launch": w["%s_hat_steps" % key],
+            "evaluated_pairs_per_launch": w["%s_pairs" % key],
+            "achieved_6d_convention": flops_6d / (dom_ms * 1e-3) / 1e12,
+        }
+
+    dom, dom_ms, r32 = roof("fp32", PEAK_TFLOPS["fp32"])
+    dom64, dom64_ms, r64 = roof("fp64", PEAK_TFLOPS["fp64"])
+    traffic, tinfo = profile_traffic(dom, "float")
+    k32 = res["fp32"]
     line = {
         "metric": "pair-evals/s (mult+multTranspose) vs FP64",
         "value": v32,
@@ -552,25 +638,24 @@ def main():
                    "data_points": M, "refinement_steps": c["steps"],
                    "launch_shape_fp32": {"mult_grid_chunks": k32["shape"][0],
                                          "multT_data_segs": k32["shape"][1]}},
-        "gflops": 6.0 * d * v32 / 1e9,
+        "gflops_6d": 6.0 * d * v32 / 1e9,
         "fp64": {"value": v64, "unit": "pair-evals/s", "ms_per_step": res["fp64"]["step_s"] * 1e3,
                  "mult_ms": res["fp64"]["mult_ms"], "multT_ms": res["fp64"]["multT_ms"],
-                 "gflops": 6.0 * d * v64 / 1e9,
-                 "tflops_vs_fp64_peak": 6.0 * d * pairs / (max(res["fp64"]["mult_ms"], res["fp64"]["multT_ms"]) * 1e-3) / 1e12 / PEAK_TFLOPS["fp64"]},
+                 "gflops_6d": 6.0 * d * v64 / 1e9,
+                 "roofline": dict(r64, bound="valu", kernel=dom64, unit="TFLOP/s")},
         "speedup_vs_fp64": v32 / v64,
         "kernel_ms_fp32": {"mult": k32["mult_ms"], "multT": k32["multT_ms"]},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved,
-                     "peak": PEAK_TFLOPS["fp32"], "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_TFLOPS["fp32"], "traffic": traffic,
-                     "flops_per_launch": flops_per_launch,
-                     "flop_convention": "6*d per pair for all M*N pairs "
-                                        "(LearnerVectorizedPerformanceCalculator); the kernels "
-                                        "evaluate only the pairs whose term can be nonzero "
-                                        "(support culling) and skip constant ModLinear factors, "
-                                        "so achieved under this convention exceeds the VALU peak; "
-                                        "the executed VALU rate is in 'executed' (rocprofv3 "
-                                        "counters)",
-                     "executed": executed},
+        "executed_work": work,
+        "roofline": dict(r32, bound="valu", kernel=dom, unit="TFLOP/s", traffic=traffic,
+                         traffic_source=tinfo,
+                         flop_convention="achieved/frac: flops the kernel executes, counted in "
+                                         "this run by the counting instantiation of the same "
+                                         "kernel (4 per hat step of an evaluated pair and "
+                                         "dimension, 2 per evaluated pair), over the kernel's "
+                                         "HIP-event time; achieved_6d_convention: 6*d flops for "
+                                         "every one of the M*N pairs "
+                                         "(LearnerVectorizedPerformanceCalculator), most of which "
+                                         "support culling skips as exact zeros"),
         "setup_s": setup_s,
     }
     if not args.no_check:
@@ -578,7 +663,36 @@ def main():
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(capi.BASIS_MODLINEAR, X, c["grid"], lv, ix, alpha, y,
                                             args.cpu_seconds)
+    del op, d_alpha, d_y, d_r, d_a
+    if not args.no_serve:
+        leg, _ = serve_leg([2000.0, 8000.0], args.serve_leg_seconds)
+        line["served"] = leg
     print(json.dumps(line))
+
+
+def profile_traffic(kernel, dtype):
+    """HBM bytes per launch of the kernel from the committed rocprofv3 counter
+    passes (profiles/traffic_<kernel>.json), used only when the profile was
+    taken of the same kernel sources (SHA-256 over the device sources)."""
+    import hashlib
+    path = os.path.join(ROOT, "profiles", "traffic_%s.json" % kernel)
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "sgpp_amd", "csrc", f), "rb") as fh:
+            h.update(fh.read())
+    digest = h.hexdigest()
+    if not os.path.exists(path):
+        return None, {"file": None, "sources_sha256": digest, "match": False}
+    with open(path) as f:
+        tj = json.load(f)
+    match = tj.get("sources_sha256") == digest
+    return (tj.get("bytes_per_launch") if match else None), {
+        "file": os.path.relpath(path, ROOT), "sources_sha256": digest,
+        "profile_sources_sha256": tj.get("sources_sha256"), "match": match,
+        "profile_kernel": tj.get("kernel")}
+
+
+KERNEL_SOURCES = ("meval_kernels.hpp", "cull_device.hpp", "meval_kernels.hip")
 
 
 if __name__ == "__main__":

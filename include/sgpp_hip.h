@@ -144,6 +144,16 @@ sgpp_hip_status sgpp_hip_meval_last_shape(const sgpp_hip_meval* h, int* mult_chu
 sgpp_hip_status sgpp_hip_meval_kernel_timing_enable(sgpp_hip_meval* h, int on);
 sgpp_hip_status sgpp_hip_meval_kernel_timing(sgpp_hip_meval* h, int* n_mult, double* ms_mult,
                                              int* n_multT, double* ms_multT);
+/* Executed work of one mult and one multTranspose in the given precision,
+ * counted by counting instantiations of the same kernels (launched untimed,
+ * on zero inputs; culling does not depend on the values). out[0], out[1]:
+ * mult hat steps (data point x grid point x evaluated dimension) and pairs
+ * (every lane of every evaluated group of grid points); out[2], out[3]: the
+ * same for multTranspose. No reference counterpart: the reference's operators
+ * evaluate every pair, so LearnerVectorizedPerformanceCalculator
+ * (datadriven/src/sgpp/datadriven/tools/LearnerVectorizedPerformanceCalculator.cpp:89-94)
+ * counts all M x N pairs. */
+sgpp_hip_status sgpp_hip_meval_count_work(sgpp_hip_meval* h, int precision, uint64_t* out4);
 sgpp_hip_status sgpp_hip_meval_destroy(sgpp_hip_meval* h);
 
 /* --------------------------------------------------------------- solver
@@ -162,7 +172,7 @@ enum { SGPP_HIP_SOLVE_FP64 = 0, SGPP_HIP_SOLVE_MIXED = 1, SGPP_HIP_SOLVE_FP32 = 
 typedef struct sgpp_hip_solve_params {
   int mode;                 /* SGPP_HIP_SOLVE_* */
   double lambda;            /* regularisation lambda (system uses M * lambda) */
-  size_t max_iterations;    /* CG iterations (per inner solve in mixed mode) */
+  size_t max_iterations;    /* CG iterations (all inner solves together in mixed mode) */
   double epsilon;           /* stop when ||r|| <= epsilon * ||r0|| (FP64 residual) */
   double inner_epsilon;     /* mixed mode: inner CG reduction per refinement step */
   size_t max_refinements;   /* mixed mode: outer iterative-refinement steps */
@@ -176,6 +186,7 @@ typedef struct sgpp_hip_solve_info {
   size_t fp32_matvecs;      /* applications of A in FP32 */
   double residual_rel;      /* final ||b - A alpha|| / ||b|| in FP64 */
   double seconds;           /* wall-clock of the solve */
+  int converged;            /* the stopping test was met (0: iteration or refinement budget spent) */
 } sgpp_hip_solve_info;
 
 /* y: M values, alpha: N values (in: start value when reuse, out: solution).
@@ -247,14 +258,19 @@ sgpp_hip_status sgpp_hip_server_admit_model(sgpp_hip_server* s, int basis, size_
                                             const double* alpha, int* model_id);
 sgpp_hip_status sgpp_hip_server_evict_model(sgpp_hip_server* s, int model_id);
 sgpp_hip_status sgpp_hip_server_memory_used(sgpp_hip_server* s, size_t* bytes);
-/* queries: q x d row-major in [0,1]^d; deadline_s <= 0 means none */
+/* queries: q x d row-major in [0,1]^d (d = the model's dimension); deadline_s
+ * <= 0 means none. An FP32 server carries each coordinate as fl32(x) plus,
+ * for models with levels where the rounding matters, fl32(x - fl32(x)). */
 sgpp_hip_status sgpp_hip_server_submit(sgpp_hip_server* s, int model_id, size_t q,
                                        const double* queries, double deadline_s,
                                        uint64_t* request_id);
 /* *status receives SGPP_HIP_REQ_*; a finished request is released (its q
- * results copied to out when DONE). wait_s < 0 blocks, 0 polls. */
+ * results copied to out when DONE). out holds out_capacity values; a DONE
+ * request with more than out_capacity results is refused with
+ * SGPP_HIP_ERR_INVALID_ARGUMENT and kept. wait_s < 0 blocks, 0 polls. */
 sgpp_hip_status sgpp_hip_server_poll(sgpp_hip_server* s, uint64_t request_id, double* out,
-                                     double wait_s, int* status, double* latency_s);
+                                     size_t out_capacity, double wait_s, int* status,
+                                     double* latency_s);
 sgpp_hip_status sgpp_hip_server_cancel(sgpp_hip_server* s, uint64_t request_id);
 /* host-forced fault: the next batch reports a device error on its requests */
 sgpp_hip_status sgpp_hip_server_inject_error(sgpp_hip_server* s);

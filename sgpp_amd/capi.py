@@ -75,7 +75,8 @@ class SolveParams(ctypes.Structure):
 class SolveInfo(ctypes.Structure):
     _fields_ = [("iterations", ctypes.c_size_t), ("refinements", ctypes.c_size_t),
                 ("fp64_matvecs", ctypes.c_size_t), ("fp32_matvecs", ctypes.c_size_t),
-                ("residual_rel", ctypes.c_double), ("seconds", ctypes.c_double)]
+                ("residual_rel", ctypes.c_double), ("seconds", ctypes.c_double),
+                ("converged", ctypes.c_int)]
 
 
 class GridConfig(ctypes.Structure):
@@ -158,6 +159,7 @@ SIGNATURES = {
     "sgpp_hip_meval_last_shape": (_st, [_p, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]),
     "sgpp_hip_meval_kernel_timing_enable": (_st, [_p, ctypes.c_int]),
+    "sgpp_hip_meval_count_work": (_st, [_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "sgpp_hip_meval_kernel_timing": (_st, [_p, ctypes.POINTER(ctypes.c_int), _dp,
                                            ctypes.POINTER(ctypes.c_int), _dp]),
     "sgpp_hip_meval_destroy": (_st, [_p]),
@@ -172,7 +174,7 @@ SIGNATURES = {
     "sgpp_hip_server_memory_used": (_st, [_p, ctypes.POINTER(_sz)]),
     "sgpp_hip_server_submit": (_st, [_p, ctypes.c_int, _sz, _dp, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_uint64)]),
-    "sgpp_hip_server_poll": (_st, [_p, ctypes.c_uint64, _dp, ctypes.c_double,
+    "sgpp_hip_server_poll": (_st, [_p, ctypes.c_uint64, _dp, _sz, ctypes.c_double,
                                    ctypes.POINTER(ctypes.c_int), _dp]),
     "sgpp_hip_server_cancel": (_st, [_p, ctypes.c_uint64]),
     "sgpp_hip_server_inject_error": (_st, [_p]),
@@ -421,6 +423,14 @@ class MultiEval:
                                                 ctypes.byref(c), ctypes.byref(d)))
         return a.value, b.value, c.value, d.value
 
+    def count_work(self, precision):
+        """Executed work of one mult and one multTranspose (counting kernels):
+        {"mult_hat_steps", "mult_pairs", "multT_hat_steps", "multT_pairs"}."""
+        out = (ctypes.c_uint64 * 4)()
+        check(F["sgpp_hip_meval_count_work"](self.h, precision, out))
+        return {"mult_hat_steps": out[0], "mult_pairs": out[1], "multT_hat_steps": out[2],
+                "multT_pairs": out[3]}
+
     def system_mult(self, v, lam, precision=PREC_FP64):
         v = _f64(v)
         q = np.empty_like(v)
@@ -482,6 +492,7 @@ class Server:
                                           ctypes.byref(h)))
         self.h = h
         self.dims = {}
+        self._q = {}  # query count of each request not yet released
 
     def close(self):
         h = getattr(self, "h", None)
@@ -513,19 +524,34 @@ class Server:
         return n.value
 
     def submit(self, model_id, queries, deadline_s=0.0):
+        """queries: q x d array, d = the model's dimension; returns (request id, q)"""
         X = _f64(queries)
+        d = self.dims.get(model_id)
+        if X.ndim != 2 or (d is not None and X.shape[1] != d):
+            raise ValueError("queries must be a q x %s array for model %d, got shape %s"
+                             % (d, model_id, X.shape))
         rid = ctypes.c_uint64(0)
         check(F["sgpp_hip_server_submit"](self.h, model_id, X.shape[0], _dptr(X), deadline_s,
                                           ctypes.byref(rid)))
+        self._q[rid.value] = X.shape[0]
         return rid.value, X.shape[0]
 
-    def poll(self, rid, q, wait_s=-1.0):
-        """(status, results or None, latency seconds); a finished request is released."""
-        out = np.empty(q)
+    def poll(self, rid, q=None, wait_s=-1.0):
+        """(status, results or None, latency seconds); a finished request is
+        released. The result buffer is sized from the request's own query
+        count (q, if given, must equal it)."""
+        n = self._q.get(rid)
+        if n is None:
+            raise SGppHIPError(8, "unknown request id %d" % rid)
+        if q is not None and q != n:
+            raise ValueError("request %d has %d queries, not %d" % (rid, n, q))
+        out = np.empty(n)
         st = ctypes.c_int(-1)
         lat = ctypes.c_double(0)
-        check(F["sgpp_hip_server_poll"](self.h, rid, _dptr(out), wait_s, ctypes.byref(st),
+        check(F["sgpp_hip_server_poll"](self.h, rid, _dptr(out), n, wait_s, ctypes.byref(st),
                                         ctypes.byref(lat)))
+        if st.value not in (REQ_QUEUED, REQ_RUNNING):
+            self._q.pop(rid, None)
         if st.value == REQ_DONE:
             return st.value, out, lat.value
         return st.value, None, lat.value

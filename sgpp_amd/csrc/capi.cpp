@@ -449,6 +449,18 @@ sgpp_hip_status sgpp_hip_meval_kernel_timing(sgpp_hip_meval* h, int* n_mult, dou
   });
 }
 
+sgpp_hip_status sgpp_hip_meval_count_work(sgpp_hip_meval* h, int precision, uint64_t* out4) {
+  return guarded([&] {
+    REQUIRE(h && out4, "null argument");
+    REQUIRE(precision == SGPP_HIP_PREC_FP64 || precision == SGPP_HIP_PREC_FP32_FP64ACC,
+            "unknown precision");
+    h->dev->count_work(precision == SGPP_HIP_PREC_FP64 ? sgpp_hip::Precision::FP64
+                                                       : sgpp_hip::Precision::FP32,
+                       out4);
+    return SGPP_HIP_OK;
+  });
+}
+
 sgpp_hip_status sgpp_hip_meval_destroy(sgpp_hip_meval* h) {
   return guarded([&] {
     delete h;
@@ -490,6 +502,7 @@ sgpp_hip_status sgpp_hip_regression_solve(sgpp_hip_meval* h, const double* y,
       info->fp32_matvecs = r.fp32_matvecs;
       info->residual_rel = r.residual_rel;
       info->seconds = r.seconds;
+      info->converged = r.converged ? 1 : 0;
     }
     return SGPP_HIP_OK;
   });

@@ -223,9 +223,10 @@ struct alignas(2 * sizeof(T)) LIPair {
 // skipped (Linear grids have no such dimension). The
 // batch's skip mask is computed from all its loaded pairs before any test, so
 // all loads of the batch are issued together, ahead of the branches.
+// Returns the number of dimensions evaluated (wave-uniform).
 template <int D, int J, int K, typename T, bool kSkip>
-__device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int (&jj)[J],
-                                           const T (&x)[K][D], T (&p)[J][K]) {
+__device__ __forceinline__ int eval_group(const T* __restrict__ tile, const int (&jj)[J],
+                                          const T (&x)[K][D], T (&p)[J][K]) {
   // dimensions per batch: the batch's pairs take at most 48 scalar registers
   constexpr int kWordsPerDim = J * 2 * static_cast<int>(sizeof(T)) / 4;
   constexpr int DB = D < 48 / kWordsPerDim ? D : 48 / kWordsPerDim;
@@ -234,6 +235,7 @@ __device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int
   for (int q = 0; q < J; ++q)
 #pragma unroll
     for (int k = 0; k < K; ++k) p[q][k] = T(1);
+  int evaluated = 0;
 #pragma unroll
   for (int d0 = 0; d0 < D; d0 += DB) {
     LIPair<T> li[J][DB];
@@ -251,6 +253,7 @@ __device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int
       for (int q = 1; q < J; ++q) any |= bits_of(li[q][dd].L) | bits_of(li[q][dd].I);
       live |= (any != 0 ? 1u : 0u) << dd;
     }
+    evaluated += __builtin_popcount(live & ((1u << (D - d0 < DB ? D - d0 : DB)) - 1u));
 #pragma unroll
     for (int dd = 0; dd < DB; ++dd) {
       const int d = d0 + dd;
@@ -261,6 +264,7 @@ __device__ __forceinline__ void eval_group(const T* __restrict__ tile, const int
         for (int k = 0; k < K; ++k) p[q][k] = hat_step(p[q][k], li[q][dd].L, x[k][d], li[q][dd].I);
     }
   }
+  return evaluated;
 }
 
 }  // namespace sgpp_hip

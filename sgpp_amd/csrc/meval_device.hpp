@@ -122,6 +122,14 @@ class DeviceMultiEval {
   void set_kernel_timing(bool on);
   void kernel_timing(int* n_mult, double* ms_mult, int* n_multT, double* ms_multT);
 
+  // Work the culled kernels execute for one mult and one multTranspose in
+  // precision p, counted by the kernels' counting instantiations (launched
+  // here, untimed, on zero inputs; culling does not depend on the values):
+  // out[0], out[1] = mult hat steps (pair x evaluated dimension) and pairs
+  // (every lane of every evaluated group); out[2], out[3] = the same for
+  // multTranspose. Synchronises the stream.
+  void count_work(Precision p, uint64_t out[4]);
+
  private:
   void ensure(void** buf, size_t* cap, size_t bytes);
   void record_start();
@@ -173,6 +181,7 @@ class DeviceMultiEval {
   };
   EventPair* next_timer(std::vector<EventPair>& pool, size_t& used);
   bool timing_on_ = false;
+  unsigned long long* count_ = nullptr;  // set while count_work launches
   std::vector<EventPair> tm_mult_, tm_multT_;
   size_t used_mult_ = 0, used_multT_ = 0;
 

@@ -46,383 +46,10 @@
 #include <numeric>
 #include <sstream>
 
-#include "cull_device.hpp"
-#include "meval_device.hpp"
+#include "meval_kernels.hpp"
 
 namespace sgpp_hip {
 
-namespace {
-
-constexpr int kBlock = 256;  // 4 waves of 64
-constexpr int kTG = 64;      // grid points per tile (culling unit, one point per lane)
-constexpr int kTM = 128;     // data points per step of the generic multTranspose
-constexpr int kMaxTemplD = 20;
-// Default launch sizes. Culled blocks cost very different amounts, so many
-// short blocks balance better: mult ~128 workgroups per CU and grid chunks of
-// at most 24 tiles; multTranspose ~256 workgroups per CU (each block covers
-// kChunkT grid points). Measured on the d=10 ModLinear (1e5 x 1e6) and d=8
-// Linear (3.2e4 x 5e5) workloads.
-constexpr int kBlocksPerCU = 128;
-constexpr int kMaxChunkTiles = 24;
-constexpr int kBlocksPerCUT = 256;
-// cap on the FP64 partials of one launch (G x Mpad or S x Npad doubles)
-constexpr int64_t kPartialBudget = int64_t(512) << 20;
-// data rows past M kept in the padded data (a wave's last group of 64*K
-// points may start just below M)
-constexpr size_t kDataSlack = 1024;
-
-// Points per lane (K) and grid points evaluated together (J). Values from
-// variant sweeps on the d=10 ModLinear, d=8 Linear and d=20 Linear workloads.
-// Kernels for d > 10 are held to 128 registers (4 waves per SIMD) although
-// some values then spill: at d = 20 that was 4-21 % faster than 3 waves
-// without spills; at d = 10 (FP64) it was 10-15 % slower, so d <= 10 keeps
-// its natural allocation.
-template <typename T>
-constexpr int mult_k(int D) {
-  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
-}
-template <typename T>
-constexpr int mult_j(int D) {
-  return sizeof(T) == 4 && D > 10 ? 4 : 2;
-}
-template <typename T>
-constexpr int multT_k(int D) {
-  return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
-}
-template <typename T>
-constexpr int multT_j(int D) {
-  return sizeof(T) == 4 && D > 10 ? 4 : 2;
-}
-// grid points per multTranspose block (FP64 accumulators in LDS, one row per wave)
-constexpr int kChunkT = 1024;
-
-// (L, I) of grid point j and dimension d: LI[(j*D + d)*2 + {0,1}] (point-major:
-// one point's 2*D values are contiguous, so a wave fetches a taken point's
-// whole row with one wide scalar load; tile t = points [64t, 64t+64)).
-__device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * D + d) * 2; }
-
-// ---------------------------------------------------------------- mult
-// Lanes own data points (K per lane, coordinates in registers). The data are
-// stored in Morton order, so a wave's 64*K points fill a small box. For every
-// 64-point grid tile of its chunk a wave
-//   1. skips the tile if the tile's box of supports misses its data box;
-//   2. tests the tile's points against the box (one point per lane, ballot);
-//   3. evaluates only points that may be nonzero in the box, J at a time,
-//      over the union of their non-constant dimensions.
-// Every dropped term is exactly zero, so the sums equal a full evaluation.
-// X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] FP32 tile
-// boxes, A: [Npad] alpha*scale (zero padded), part: [G][ldx]. Block: kBlock*K
-// data points (each wave 64*K consecutive ones) x grid chunk [y*chunk,
-// +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
-template <int D, int K, int J, typename T, bool kSkip>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
-void mult_kernel(const T* __restrict__ X, int64_t ldx,
-                                                      const T* __restrict__ LI,
-                                                      const float* __restrict__ TB,
-                                                      const T* __restrict__ A,
-                                                      double* __restrict__ part, int chunk,
-                                                      int jend_all) {
-  // a wave's 64*K points are contiguous (a small box in Morton order): point
-  // k of a lane is m0 + 64*k
-  const int lane = threadIdx.x & 63;
-  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + (threadIdx.x >> 6) * (64 * K) + lane;
-  T x[K][D];
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-#pragma unroll
-    for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * 64];
-  WaveBox<D, T> box;
-  wave_box<D, D, K, T>(x, box);
-  double accd[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) accd[k] = 0.0;
-  const int jb = blockIdx.y * chunk;
-  const int je = min(jb + chunk, jend_all);
-  for (int j0 = jb; j0 < je; j0 += kTG) {
-    const int64_t t = j0 / kTG;
-    if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
-    const T* tile = LI + t * D * kTG * 2;
-    unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
-    T acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] = T(0);
-    while (bal) {
-      int jj[J];
-      bool used[J];
-      T a[J];
-      take_hits<J, true, T>(bal, A, j0, jj, used, a);
-      T p[J][K];
-      eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
-#pragma unroll
-      for (int q = 0; q < J; ++q)
-#pragma unroll
-        for (int k = 0; k < K; ++k) acc[k] = fmaT(a[q], p[q][k], acc[k]);
-    }
-#pragma unroll
-    for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
-  }
-  double* out = part + static_cast<int64_t>(blockIdx.y) * ldx;
-#pragma unroll
-  for (int k = 0; k < K; ++k) out[m0 + k * 64] = accd[k];
-}
-
-// Generic-dimension mult (D > kMaxTemplD): one data point per lane,
-// coordinates re-read from global memory (cached), runtime loops, no culling.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restrict__ X, int64_t ldx,
-                                                              const T* __restrict__ LI,
-                                                              const T* __restrict__ A,
-                                                              double* __restrict__ part, int chunk,
-                                                              int jend_all, int D) {
-  const int64_t m = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int jb = blockIdx.y * chunk;
-  const int je = min(jb + chunk, jend_all);
-  double accd = 0.0;
-  for (int j0 = jb; j0 < je; j0 += kTG) {
-    T acc = T(0);
-    for (int j = j0; j < j0 + kTG; ++j) {
-      T p = T(1);
-      for (int d = 0; d < D; ++d) {
-        const int64_t e = li_at(j, d, D);
-        p = hat_step(p, LI[e], X[d * ldx + m], LI[e + 1]);
-      }
-      acc = fmaT(A[j], p, acc);
-    }
-    accd += static_cast<double>(acc);
-  }
-  part[static_cast<int64_t>(blockIdx.y) * ldx + m] = accd;
-}
-
-// ------------------------------------------------------- multTranspose
-// Same orientation as mult: lanes own data points (K per lane, coordinates
-// and y in registers), grid tiles are culled against the wave's data box and
-// the surviving points are evaluated J at a time. A point's contribution
-// sum_k y_k phi(x_k) is summed over the wave's lanes (DPP) into lane jj of a
-// per-tile register (a point is taken at most once per group and tile); the
-// tile is then added, in FP64, to the wave's own row of LDS accumulators. At
-// the end the four rows are added in a fixed order. Block: data segment [blockIdx.x*seg, +seg) x
-// grid chunk [blockIdx.y*kChunkT, +kChunkT), the segment split into four
-// contiguous wave shares. part: [S][ldp].
-template <int D, int K, int J, typename T, bool kSkip>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
-void multT_kernel(const T* __restrict__ X, int64_t ldx,
-                                                       const T* __restrict__ Y,
-                                                       const T* __restrict__ LI,
-                                                       const float* __restrict__ TB,
-                                                       double* __restrict__ part, int64_t ldp,
-                                                       int jend_all, int64_t seg, int64_t mend) {
-  __shared__ double accs[kBlock / 64][kChunkT];
-  const int wave = threadIdx.x >> 6;
-  const int lane = threadIdx.x & 63;
-  const int jb = blockIdx.y * kChunkT;
-  const int je = min(jb + kChunkT, jend_all);
-  for (int i = threadIdx.x; i < (kBlock / 64) * kChunkT; i += kBlock) (&accs[0][0])[i] = 0.0;
-  __syncthreads();
-  const int64_t mb = static_cast<int64_t>(blockIdx.x) * seg;
-  const int64_t me = min(mb + seg, mend);
-  // per-wave share, a multiple of 64*K; points past the share weigh 0
-  const int64_t share = ((me - mb) / (kBlock / 64) + 64 * K - 1) / (64 * K) * (64 * K);
-  const int64_t wb = mb + wave * share;
-  const int64_t we = min(wb + share, me);
-  double* acc = accs[wave];
-  for (int64_t m0 = wb; m0 < we; m0 += 64 * K) {
-    T x[K][D], y[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const int64_t m = m0 + k * 64 + lane;
-#pragma unroll
-      for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m];
-      y[k] = (m < we) ? Y[m] : T(0);
-    }
-    WaveBox<D, T> box;
-    wave_box<D, D, K, T>(x, box);
-    for (int j0 = jb; j0 < je; j0 += kTG) {
-      const int64_t t = j0 / kTG;
-      if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
-      const T* tile = LI + t * D * kTG * 2;
-      unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
-      // a grid point is taken at most once per (group, tile): its sum goes
-      // to lane jj of tsum, and the tile is added to the wave's row once
-      T tsum = T(0);
-      while (bal) {
-        int jj[J];
-        bool used[J];
-        T unused_w[J];
-        take_hits<J, false, T>(bal, static_cast<const T*>(nullptr), j0, jj, used, unused_w);
-        T p[J][K];
-        eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
-#pragma unroll
-        for (int q = 0; q < J; ++q) {
-          if (!used[q]) continue;
-          T c = T(0);
-#pragma unroll
-          for (int k = 0; k < K; ++k) c = fmaT(y[k], p[q][k], c);
-          const T sq = lane63(sum_to_lane63(c));
-          tsum = lane == jj[q] ? sq : tsum;
-        }
-      }
-      // one vector add per tile into the wave's own accumulator row (in
-      // issue order: deterministic; untaken lanes add +0)
-      atomicAdd(&acc[j0 - jb + lane], static_cast<double>(tsum));
-    }
-  }
-  __syncthreads();
-  double* out = part + static_cast<int64_t>(blockIdx.x) * ldp;
-  for (int i = threadIdx.x; i < je - jb; i += kBlock) {
-    double s = accs[0][i];
-#pragma unroll
-    for (int w = 1; w < kBlock / 64; ++w) s += accs[w][i];
-    out[jb + i] = s;
-  }
-}
-
-template <typename T>
-__global__ __launch_bounds__(kBlock) void multT_kernel_generic(const T* __restrict__ X, int64_t ldx,
-                                                               const T* __restrict__ Y,
-                                                               const T* __restrict__ LI,
-                                                               double* __restrict__ part,
-                                                               int64_t ldp, int64_t seg,
-                                                               int64_t mend, int D) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const int64_t mb = static_cast<int64_t>(blockIdx.y) * seg;
-  const int64_t me = min(mb + seg, mend);
-  double accd = 0.0;
-  for (int64_t m0 = mb; m0 < me; m0 += kTM) {
-    T acc = T(0);
-    for (int64_t m = m0; m < m0 + kTM; ++m) {
-      T p = T(1);
-      for (int d = 0; d < D; ++d) {
-        const int64_t e = li_at(j, d, D);
-        p = hat_step(p, LI[e], X[d * ldx + m], LI[e + 1]);
-      }
-      acc = fmaT(Y[m], p, acc);
-    }
-    accd += static_cast<double>(acc);
-  }
-  part[static_cast<int64_t>(blockIdx.y) * ldp + j] = accd;
-}
-
-// ------------------------------------------------------------- helpers
-// A[s] = alpha[perm[s]] * scale[s] for stored position s (perm may be null)
-template <typename T>
-__global__ void prep_alpha_kernel(const double* __restrict__ alpha, const double* __restrict__ scale,
-                                  const int32_t* __restrict__ perm, T* __restrict__ A, int64_t n,
-                                  int64_t npad) {
-  const int64_t s = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (s >= npad) return;
-  if (s < n) {
-    const int64_t j = perm ? perm[s] : s;
-    A[s] = static_cast<T>(alpha[j] * scale[s]);
-  } else {
-    A[s] = T(0);
-  }
-}
-
-// Y[m] = y[dperm[m]] for stored position m (data in storage order), 0 past n
-template <typename T>
-__global__ void prep_y_kernel(const double* __restrict__ y, const int32_t* __restrict__ dperm,
-                              T* __restrict__ Y, int64_t n, int64_t npad) {
-  const int64_t m = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (m < npad) Y[m] = (m < n) ? static_cast<T>(y[dperm[m]]) : T(0);
-}
-
-// out[perm ? perm[i] : i] = (scale ? scale[i] : 1) * sum_{p < nparts} part[p*ld + i],
-// summed in the fixed order p = 0, 1, ... Each thread owns the pair i0, i0 + 1
-// (16-byte loads; ld is even and the rows are padded, so i0 + 1 < ld) and
-// issues eight rows' loads before their adds.
-__global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld, int nparts,
-                                    const double* __restrict__ scale,
-                                    const int32_t* __restrict__ perm, double* __restrict__ out,
-                                    int64_t n) {
-  const int64_t i0 = 2 * (static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x);
-  if (i0 >= n) return;
-  const double2* col = reinterpret_cast<const double2*>(part + i0);
-  const int64_t ld2 = ld / 2;
-  double s0 = 0.0, s1 = 0.0;
-  int p = 0;
-  for (; p + 8 <= nparts; p += 8) {
-    double2 v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) v[u] = col[(p + u) * ld2];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      s0 += v[u].x;
-      s1 += v[u].y;
-    }
-  }
-  for (; p < nparts; ++p) {
-    const double2 v = col[p * ld2];
-    s0 += v.x;
-    s1 += v.y;
-  }
-  out[perm ? perm[i0] : i0] = scale ? s0 * scale[i0] : s0;
-  if (i0 + 1 < n) out[perm ? perm[i0 + 1] : i0 + 1] = scale ? s1 * scale[i0 + 1] : s1;
-}
-
-// ------------------------------------------------------------ dispatch
-// kSkip: test for constant dimensions (ModLinear); Linear grids have none
-template <typename T, int D, bool kSkip>
-void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const T* A,
-                   double* part, int chunk, int jend, int G, int64_t Mpad,
-                   hipStream_t s) {
-  constexpr int K = mult_k<T>(D);
-  dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
-  clear_launch_error();
-  hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T, kSkip>), grid, dim3(kBlock), 0, s, X,
-                     ldx, LI, TB, A, part, chunk, jend);
-}
-
-template <typename T, int D, bool kSkip>
-void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const float* TB,
-                    double* part, int64_t ldp, int jend, int64_t seg,
-                    int64_t mend, int S, hipStream_t s) {
-  constexpr int K = multT_k<T>(D);
-  // data segments vary fastest: consecutive blocks share one grid chunk
-  // (its rows stay hot in every XCD's caches) and read different data
-  dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>((jend + kChunkT - 1) / kChunkT));
-  clear_launch_error();
-  hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip>), grid, dim3(kBlock), 0, s, X,
-                     ldx, Y, LI, TB, part, ldp, jend, seg, mend);
-}
-
-template <typename T>
-using MultFn = void (*)(const T*, int64_t, const T*, const float*, const T*, double*, int, int,
-                        int, int64_t, hipStream_t);
-template <typename T>
-using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, double*, int64_t,
-                         int, int64_t, int64_t, int, hipStream_t);
-
-template <typename T, int... Ds>
-struct Table {
-  static MultFn<T> mult(int D, bool skip) {
-    MultFn<T> f = nullptr;
-    ((D == Ds ? (f = skip ? &launch_mult_D<T, Ds, true> : &launch_mult_D<T, Ds, false>, 0) : 0),
-     ...);
-    return f;
-  }
-  static MultTFn<T> multT(int D, bool skip) {
-    MultTFn<T> f = nullptr;
-    ((D == Ds ? (f = skip ? &launch_multT_D<T, Ds, true> : &launch_multT_D<T, Ds, false>, 0)
-              : 0),
-     ...);
-    return f;
-  }
-};
-template <typename T>
-using DimTable = Table<T, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20>;
-
-template <typename T>
-int mult_k_rt(int D) {
-  return D > kMaxTemplD ? 1 : mult_k<T>(D);
-}
-template <typename T>
-int multT_k_rt(int D) {
-  return D > kMaxTemplD ? 1 : multT_k<T>(D);
-}
-
-int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
-
-}  // namespace
 
 // Morton order of n points in [0,1]^d (row-major): b = min(16, 64/d) bits
 // per coordinate, interleaved from the top bit; ties keep the input order.
@@ -719,9 +346,11 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
   double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
-    MultFn<T> f = DimTable<T>::mult(static_cast<int>(D_), basis_ == Basis::ModLinear);
+    const bool skip = basis_ == Basis::ModLinear;
+    MultFn<T> f = count_ ? count_mult_launcher<T>(static_cast<int>(D_), skip)
+                         : DimTable<T>::mult(static_cast<int>(D_), skip);
     f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A_),
-      P, chunk, jend, G, static_cast<int64_t>(Mpad_), s);
+      P, chunk, jend, G, static_cast<int64_t>(Mpad_), count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();
@@ -777,10 +406,11 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
   double* P = static_cast<double*>(part_);
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (templ) {
-    MultTFn<T> f = DimTable<T>::multT(static_cast<int>(D_), basis_ == Basis::ModLinear);
+    const bool skip = basis_ == Basis::ModLinear;
+    MultTFn<T> f = count_ ? count_multT_launcher<T>(static_cast<int>(D_), skip)
+                          : DimTable<T>::multT(static_cast<int>(D_), skip);
     f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
-      P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
-      s);
+      P, static_cast<int64_t>(Npad_), jend, seg, mend, S, count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(S));
     clear_launch_error();
@@ -829,6 +459,48 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
     run_multT<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
                      static_cast<const float*>(TB32_), d_y, d_result, tp ? tev : nullptr);
   record_stop();
+}
+
+void DeviceMultiEval::count_work(Precision p, uint64_t out[4]) {
+  if (N_ == 0) throw input_error("grid not set (call prepare)");
+  if (!serves(p)) throw input_error("precision not enabled for this instance");
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  const size_t n = std::max(N_, M_) * sizeof(double);
+  ensure(&hin_, &hin_cap_, n);
+  ensure(&hout_, &hout_cap_, n);
+  check_hip(hipMemsetAsync(hin_, 0, n, stream_), "clear count input");
+  unsigned long long* dcnt = nullptr;
+  check_hip(hipMalloc(reinterpret_cast<void**>(&dcnt), 4 * sizeof(unsigned long long)),
+            "hipMalloc(counters)");
+  const bool timing = timing_on_;
+  timing_on_ = false;  // counting launches are never timed
+  try {
+    check_hip(hipMemsetAsync(dcnt, 0, 4 * sizeof(unsigned long long), stream_), "clear counters");
+    count_ = dcnt;
+    mult_device(static_cast<const double*>(hin_), static_cast<double*>(hout_), p);
+    mult_transpose_device(static_cast<const double*>(hin_), static_cast<double*>(hout_), p);
+    count_ = nullptr;
+    unsigned long long h[4];
+    check_hip(hipMemcpyAsync(h, dcnt, sizeof(h), hipMemcpyDeviceToHost, stream_), "read counters");
+    check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+    for (int i = 0; i < 4; ++i) out[i] = h[i];
+  } catch (...) {
+    count_ = nullptr;
+    timing_on_ = timing;
+    (void)hipStreamSynchronize(stream_);
+    (void)hipFree(dcnt);
+    throw;
+  }
+  timing_on_ = timing;
+  check_hip(hipFree(dcnt), "hipFree(counters)");
+  if (D_ > static_cast<size_t>(kMaxTemplD)) {
+    // the generic kernels evaluate every pair of the padded arrays in every dimension
+    const uint64_t jend = static_cast<uint64_t>(ceil_div(N_, kTG)) * kTG;
+    out[1] = static_cast<uint64_t>(Mpad_) * jend;
+    out[0] = out[1] * D_;
+    out[3] = out[1];
+    out[2] = out[0];
+  }
 }
 
 DeviceMultiEval::EventPair* DeviceMultiEval::next_timer(std::vector<EventPair>& pool,

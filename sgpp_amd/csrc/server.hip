@@ -28,9 +28,18 @@ struct ModelDesc {
   const void* A;          // [Npad] alpha * scale, zero padded
   const uint32_t* qmask;  // [Npad/4] dimensions evaluated per 4 consecutive points
   const uint32_t* umask;  // [Npad/J] dimensions whose (L, I) a sub-tile's J points share
+  const uint32_t* hmask;  // [Npad/J] dimensions in which a sub-tile point has L >= kLoCorrectL
   int D;
   int Npad;               // multiple of kTG
 };
+
+// FP32 serving: a query coordinate x (FP64) is carried as hi = fl32(x) and
+// lo = fl32(x - hi). fma(L, hi, -I) is exact inside the support (Sterbenz for
+// I >= 2; for I <= 1 the one rounding is below 2^-25), so the only input
+// error of u = L x - I is L * lo, up to L * 2^-25. Factors with
+// L >= kLoCorrectL add it back, u = fma(L, lo, fma(L, hi, -I)); below that
+// the dropped term is under 2^-21 per factor.
+constexpr float kLoCorrectL = 32.0f;
 
 // The queries of one model in a batch ("span"): stored [D][ldq] at xoff and
 // cut into ldq / tile query tiles. Its work items are the (query tile, grid
@@ -47,7 +56,7 @@ struct Span {
   int ldq;     // padded query count, a multiple of tile
   int item0;
   int slot0;
-  int pad;
+  int lo;      // 1: the low parts of the queries follow at xoff + D * ldq
   long long xoff;
   long long pbase;
   long long out0;  // output offset of the span's first query
@@ -59,6 +68,7 @@ struct Item {
   int q0;          // first query of the tile (span-relative)
   int j0, j1;      // grid chunk, multiples of kTG
   int ldq;
+  int lo;          // low parts present (FP32 queries of a model with hmask bits)
   long long xoff;
   long long poff;  // double offset of the item's partial row
 };
@@ -96,9 +106,9 @@ constexpr int serve_j(int D) {
 }
 
 // LDS staging buffer of one 64-point grid tile: weights A (kTG values),
-// per-4-point masks (kTG/4 words), shared-dimension masks per J-point
-// sub-tile (kTG/J <= kTG/4 words), then the tile-major (L, I) pairs
-// (D*kTG*2 values); every part starts on a 16-byte boundary.
+// per-4-point masks (kTG/4 words), shared-dimension and low-part masks per
+// J-point sub-tile (kTG/J <= kTG/4 words each), then the tile-major (L, I)
+// pairs (D*kTG*2 values); every part starts on a 16-byte boundary.
 template <typename T>
 constexpr int stage_a_off() { return 0; }
 template <typename T>
@@ -106,7 +116,9 @@ constexpr int stage_m_off() { return kTG * static_cast<int>(sizeof(T)); }
 template <typename T>
 constexpr int stage_u_off() { return stage_m_off<T>() + kTG; }
 template <typename T>
-constexpr int stage_li_off() { return stage_u_off<T>() + kTG; }
+constexpr int stage_h_off() { return stage_u_off<T>() + kTG; }
+template <typename T>
+constexpr int stage_li_off() { return stage_h_off<T>() + kTG; }
 template <typename T>
 constexpr int stage_bytes(int D) { return stage_li_off<T>() + kTG * D * 2 * static_cast<int>(sizeof(T)); }
 
@@ -134,6 +146,8 @@ __device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned
               kTG / 16, wave, lane);
   copy_to_lds(reinterpret_cast<const uint4*>(md.umask + j0 / J),
               reinterpret_cast<uint4*>(buf + stage_u_off<T>()), kTG / J / 4, wave, lane);
+  copy_to_lds(reinterpret_cast<const uint4*>(md.hmask + j0 / J),
+              reinterpret_cast<uint4*>(buf + stage_h_off<T>()), kTG / J / 4, wave, lane);
   copy_to_lds(reinterpret_cast<const uint4*>(LI + static_cast<long long>(j0) * D * 2 * sizeof(T)),
               reinterpret_cast<uint4*>(buf + stage_li_off<T>()), kTG * D * 2 * sizeof(T) / 16,
               wave, lane);
@@ -147,12 +161,17 @@ __device__ __forceinline__ void stage_tile(const ModelDesc& md, int j0, unsigned
 // skipped (ModLinear level-1 factors are exactly 1; Linear models carry full
 // masks), evaluated once per query because all J points share its (L, I)
 // (the factor then scales the sub-tile's weighted sum), or evaluated per
-// point.
+// point. FP32 items with low parts add L * lo in the dimensions of the
+// sub-tile's low-part mask (lo is read from the batch buffer there, so it
+// holds no registers across the loop).
 template <int D, int K, int J, typename T>
 __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md, const T* X,
                                            double* part, unsigned char* sbuf, int bufbytes) {
   static_assert(kTG % J == 0 && J % 4 == 0, "sub-tile must divide the tile, 4 must divide it");
+  constexpr bool kLo = sizeof(T) == 4;
   const T* Xr = X + item.xoff;
+  const T* Xl = Xr + static_cast<long long>(D) * item.ldq;  // low parts (when item.lo)
+  const bool lo = kLo && item.lo != 0;
   T x[K][D];
 #pragma unroll
   for (int k = 0; k < K; ++k)
@@ -172,6 +191,7 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
     const T* sA = reinterpret_cast<const T*>(buf + stage_a_off<T>());
     const uint32_t* sM = reinterpret_cast<const uint32_t*>(buf + stage_m_off<T>());
     const uint32_t* sU = reinterpret_cast<const uint32_t*>(buf + stage_u_off<T>());
+    const uint32_t* sH = reinterpret_cast<const uint32_t*>(buf + stage_h_off<T>());
     const T* sLI = reinterpret_cast<const T*>(buf + stage_li_off<T>());
     T acc[K];
 #pragma unroll
@@ -182,6 +202,7 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
       for (int c = 0; c < J / 4; ++c) mask |= sM[j / 4 + c];
       mask = __builtin_amdgcn_readfirstlane(mask);
       const uint32_t shared = __builtin_amdgcn_readfirstlane(sU[j / J]);
+      const uint32_t hm = lo ? __builtin_amdgcn_readfirstlane(sH[j / J]) : 0u;
       T p[J][K];
       T g[K];  // product of the factors the J points share
 #pragma unroll
@@ -194,10 +215,11 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
       // sub-tile shares none), then the per-point ones: two simple loops
       // compile to fewer scalar instructions than one with both cases
       const uint32_t msh = mask & shared, mrest = mask & ~shared;
-      if (msh != 0u) {
+      const uint32_t msh_plain = msh & ~hm, mplain = mrest & ~hm;
+      if (msh_plain != 0u) {
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-          if (!((msh >> d) & 1u)) continue;
+          if (!((msh_plain >> d) & 1u)) continue;
           const T L = sLI[(d * kTG + j) * 2], I = sLI[(d * kTG + j) * 2 + 1];
 #pragma unroll
           for (int k = 0; k < K; ++k) g[k] *= clamp01(T(1) - absT(fmaT(L, x[k][d], -I)));
@@ -205,7 +227,7 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        if (!((mrest >> d) & 1u)) continue;
+        if (!((mplain >> d) & 1u)) continue;
         T li[2 * J];
 #pragma unroll
         for (int c = 0; c < 2 * J; ++c) li[c] = sLI[(d * kTG + j) * 2 + c];
@@ -216,6 +238,42 @@ __device__ __forceinline__ void serve_item(const Item& item, const ModelDesc& md
             const T u = fmaT(li[2 * q], x[k][d], -li[2 * q + 1]);
             p[q][k] = clamp01(fmaT(-p[q][k], absT(u), p[q][k]));
           }
+      }
+      // dimensions that add the low parts: a scalar loop over the mask bits
+      // (few per sub-tile), hi and lo re-read from the batch buffer, two
+      // queries per lane at a time (registers)
+      if (kLo && (mask & hm) != 0u) {
+        constexpr int KC = K < 2 ? K : 2;
+        for (uint32_t mc = mask & hm; mc != 0u; mc &= mc - 1u) {
+          const int d = __builtin_ctz(mc);
+          const long long off = static_cast<long long>(d) * item.ldq + item.q0 + threadIdx.x;
+          const bool sh = (shared >> d) & 1u;
+#pragma unroll
+          for (int k0 = 0; k0 < K; k0 += KC) {
+            T xh[KC], xl[KC];
+#pragma unroll
+            for (int k = 0; k < KC; ++k) {
+              xh[k] = Xr[off + (k0 + k) * kBlock];
+              xl[k] = Xl[off + (k0 + k) * kBlock];
+            }
+            if (sh) {
+              const T L = sLI[(d * kTG + j) * 2], I = sLI[(d * kTG + j) * 2 + 1];
+#pragma unroll
+              for (int k = 0; k < KC; ++k)
+                g[k0 + k] *= clamp01(T(1) - absT(fmaT(L, xl[k], fmaT(L, xh[k], -I))));
+            } else {
+#pragma unroll
+              for (int q = 0; q < J; ++q) {
+                const T L = sLI[(d * kTG + j + q) * 2], I = sLI[(d * kTG + j + q) * 2 + 1];
+#pragma unroll
+                for (int k = 0; k < KC; ++k) {
+                  const T u = fmaT(L, xl[k], fmaT(L, xh[k], -I));
+                  p[q][k0 + k] = clamp01(fmaT(-p[q][k0 + k], absT(u), p[q][k0 + k]));
+                }
+              }
+            }
+          }
+        }
       }
       T sub[K];
 #pragma unroll
@@ -284,6 +342,7 @@ void serve_kernel(const ModelDesc* __restrict__ models, const Span* __restrict__
     item.j0 = (local - t * sp.nchunks) * sp.chunk;
     item.j1 = min(item.j0 + sp.chunk, md.Npad);
     item.ldq = sp.ldq;
+    item.lo = sp.lo;
     item.xoff = sp.xoff;
     item.poff = sp.pbase + static_cast<long long>(local) * sp.tile;
     switch (sp.D) {
@@ -341,12 +400,13 @@ __global__ void reset_counter(unsigned* counter) {
 }
 
 // Span of q queries of one model (stored [D][ldq] at xoff, ldq a multiple of
-// tile), its items numbered from item0, its slots from slot0, its partials at
-// pbase and its results at out0.
+// tile, followed by their [D][ldq] low parts when lo), its items numbered from
+// item0, its slots from slot0, its partials at pbase and its results at out0.
 Span make_span(int table_index, int D, size_t Npad, size_t tile, size_t chunk, size_t q,
                size_t ldq, long long xoff, size_t item0, size_t slot0, size_t pbase,
-               long long out0) {
+               long long out0, bool lo) {
   Span sp{};
+  sp.lo = lo ? 1 : 0;
   sp.model = table_index;
   sp.D = D;
   sp.tile = static_cast<int>(tile);
@@ -392,13 +452,16 @@ struct Server::Model {
   void* A = nullptr;
   void* pm = nullptr;  // [Npad/4] dimensions evaluated per 4 consecutive points
   void* um = nullptr;  // [Npad/J] dimensions whose (L, I) the J points of a sub-tile share
+  void* hm = nullptr;  // [Npad/J] dimensions of a sub-tile that add the queries' low parts
+  bool lo = false;     // FP32 server and some hm bit set: queries carry low parts
   size_t bytes = 0;
-  int active = 0;  // requests queued or running
+  int active = 0;    // requests queued or running
+  int inflight = 0;  // pieces of its requests in formed or running batches
   void free_device() {
-    void* p[] = {LI, A, pm, um};
+    void* p[] = {LI, A, pm, um, hm};
     for (void* q : p)
       if (q) (void)hipFree(q);
-    LI = A = pm = um = nullptr;
+    LI = A = pm = um = hm = nullptr;
   }
 };
 
@@ -408,7 +471,7 @@ struct Server::Request {
   size_t q = 0;
   size_t next = 0;      // first query not yet placed in a batch
   size_t done = 0;      // queries whose results have arrived
-  std::vector<char> X;  // [D][q] in the compute type
+  std::vector<char> X;  // [D][q] in the compute type; FP32: then the [D][q] low parts
   std::vector<double> out;
   double t_submit = 0.0, t_done = 0.0, deadline = 0.0;
   int status = REQ_QUEUED;  // QUEUED while queries remain unplaced
@@ -447,6 +510,7 @@ struct Server::Slot {
     long long xoff;
     int D;
     size_t q, ldq;
+    bool lo;
   };
   std::vector<Block> blocks;  // one per model in the batch (padding columns)
   size_t usedX = 0, nItems = 0, nSlots = 0, nSpans = 0, usedPart = 0, usedOut = 0;
@@ -596,7 +660,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
   const size_t J = static_cast<size_t>(serve_j_rt(cfg_.precision, m->D));
-  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz + (m->Npad / 4 + m->Npad / J) * sizeof(uint32_t);
+  m->bytes = m->Npad * dim * 2 * esz + m->Npad * esz +
+             (m->Npad / 4 + 2 * (m->Npad / J)) * sizeof(uint32_t);
   std::vector<double> A(m->Npad, 0.0);
   for (size_t s2 = 0; s2 < N; ++s2) A[s2] = alpha[order[s2]] * sc[s2];
   // pairs as the kernel sees them (rounded to the compute type)
@@ -631,6 +696,19 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     }
     um[t] = u;
   }
+  // per sub-tile (FP32 only): dimensions in which some real point has
+  // L >= kLoCorrectL, where the queries' low parts are added
+  std::vector<uint32_t> hmk(m->Npad / J, 0u);
+  if (!f64)
+    for (size_t t = 0; t * J < N; ++t) {
+      const size_t j0 = t * J, j1 = std::min(N, j0 + J);
+      uint32_t h = 0;
+      for (size_t d = 0; d < dim; ++d)
+        for (size_t j = j0; j < j1; ++j)
+          if (pair_at(j, d, 0) >= static_cast<double>(kLoCorrectL)) h |= 1u << d;
+      hmk[t] = h;
+      if (h) m->lo = true;
+    }
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (used_bytes_ + m->bytes > cfg_.memory_budget)
@@ -662,6 +740,9 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     check_hip(hipMalloc(&m->um, um.size() * sizeof(uint32_t)), "hipMalloc(model shared masks)");
     check_hip(hipMemcpy(m->um, um.data(), um.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
               "upload shared masks");
+    check_hip(hipMalloc(&m->hm, hmk.size() * sizeof(uint32_t)), "hipMalloc(model low-part masks)");
+    check_hip(hipMemcpy(m->hm, hmk.data(), hmk.size() * sizeof(uint32_t), hipMemcpyHostToDevice),
+              "upload low-part masks");
     if (f64) {
       check_hip(hipMemcpy(m->LI, li.data(), li.size() * 8, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, A.data(), A.size() * 8, hipMemcpyHostToDevice), "upload A");
@@ -671,7 +752,8 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       check_hip(hipMemcpy(m->A, a32.data(), a32.size() * 4, hipMemcpyHostToDevice), "upload A");
     }
     ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->pm),
-                 static_cast<const uint32_t*>(m->um), m->D, static_cast<int>(m->Npad)};
+                 static_cast<const uint32_t*>(m->um), static_cast<const uint32_t*>(m->hm), m->D,
+                 static_cast<int>(m->Npad)};
     check_hip(hipMemcpy(static_cast<ModelDesc*>(model_table_) + m->table_index, &md,
                         sizeof(ModelDesc), hipMemcpyHostToDevice),
               "upload model descriptor");
@@ -714,7 +796,9 @@ void Server::evict_model(int id) {
     std::lock_guard<std::mutex> lk(mu_);
     auto it = models_.find(id);
     if (it == models_.end()) throw input_error("unknown model");
-    if (it->second->active > 0) throw input_error("model has queued or running requests");
+    // pieces of a cancelled request can still sit in a formed or running batch
+    if (it->second->active > 0 || it->second->inflight > 0)
+      throw input_error("model has queued or running requests");
     m = it->second;
     models_.erase(it);
     used_bytes_ -= m->bytes;
@@ -753,14 +837,20 @@ uint64_t Server::submit(int model, size_t q, const double* queries, double deadl
     if (!(queries[i] >= 0.0 && queries[i] <= 1.0))
       throw input_error("query points must lie in [0,1]^d");
   const bool f64 = cfg_.precision == Precision::FP64;
-  r->X.resize(D * q * (f64 ? 8 : 4));
+  // FP32: hi = fl32(x) and, for models that add them, lo = fl32(x - hi)
+  r->X.resize(D * q * (f64 ? 8 : (m->lo ? 8 : 4)));
   for (size_t i = 0; i < q; ++i)
     for (size_t d = 0; d < D; ++d) {
       const double v = queries[i * D + d];
-      if (f64)
+      if (f64) {
         reinterpret_cast<double*>(r->X.data())[d * q + i] = v;
-      else
-        reinterpret_cast<float*>(r->X.data())[d * q + i] = static_cast<float>(v);
+      } else {
+        const float hi = static_cast<float>(v);
+        reinterpret_cast<float*>(r->X.data())[d * q + i] = hi;
+        if (m->lo)
+          reinterpret_cast<float*>(r->X.data())[(D + d) * q + i] =
+              static_cast<float>(v - static_cast<double>(hi));
+      }
     }
   r->t_submit = t_enter;  // latency includes the conversion of the queries
   r->deadline = deadline_s > 0 ? r->t_submit + deadline_s : 0.0;
@@ -777,7 +867,8 @@ uint64_t Server::submit(int model, size_t q, const double* queries, double deadl
   return r->id;
 }
 
-int Server::poll(uint64_t id, double* out, double wait_s, double* latency_s) {
+int Server::poll(uint64_t id, double* out, size_t out_capacity, double wait_s,
+                 double* latency_s) {
   std::unique_lock<std::mutex> lk(mu_);
   auto it = requests_.find(id);
   if (it == requests_.end()) return -1;
@@ -790,7 +881,11 @@ int Server::poll(uint64_t id, double* out, double wait_s, double* latency_s) {
       done_cv_.wait_for(lk, std::chrono::duration<double>(wait_s), finished);
   }
   if (!finished()) return r->status;
-  if (r->status == REQ_DONE && out) std::memcpy(out, r->out.data(), r->q * sizeof(double));
+  if (r->status == REQ_DONE && out) {
+    // the request stays available for a poll with a large enough buffer
+    if (out_capacity < r->q) throw input_error("output buffer holds fewer values than the request");
+    std::memcpy(out, r->out.data(), r->q * sizeof(double));
+  }
   if (latency_s) *latency_s = r->t_done - r->t_submit;
   requests_.erase(id);
   return r->status;
@@ -831,9 +926,12 @@ void Server::fail_request(const std::shared_ptr<Request>& r, int status, const c
 // fail (a request split over several batches fails once, at its first failed
 // piece) and the slot is free again
 void Server::fail_pieces(Slot& s, const char* why) {
-  for (const Piece& pc : s.pieces)
+  for (const Piece& pc : s.pieces) {
     if (pc.r->status == REQ_QUEUED || pc.r->status == REQ_RUNNING)
       fail_request(pc.r, REQ_FAILED_DEVICE, why);
+    auto m = models_.find(pc.r->model);
+    if (m != models_.end()) m->second->inflight--;
+  }
   s.pieces.clear();
 }
 
@@ -858,7 +956,7 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
   // resources of a block of q queries
   auto need = [](const Group& g, size_t q, size_t* xel, size_t* items, size_t* part) {
     const size_t ntiles = (q + g.tile - 1) / g.tile;
-    *xel = static_cast<size_t>(g.m->D) * ntiles * g.tile;
+    *xel = static_cast<size_t>(g.m->D) * ntiles * g.tile * (g.m->lo ? 2 : 1);
     *items = ntiles * g.nchunks;
     *part = *items * g.tile;
   };
@@ -924,6 +1022,7 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
     gg.q += n;
     gg.pieces.push_back(taken.size());
     taken.push_back(pc);
+    models_.at(r->model)->inflight++;
     r->next += n;
     if (r->next == r->q) {
       queue_.pop_front();
@@ -943,12 +1042,12 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
       pc.out_off = out0 + static_cast<long long>(pc.qoff);
     }
     s.hSpans[s.nSpans++] = make_span(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q,
-                                     ldq, xoff, s.nItems, s.nSlots, s.usedPart, out0);
-    s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq});
+                                     ldq, xoff, s.nItems, s.nSlots, s.usedPart, out0, g.m->lo);
+    s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq, g.m->lo});
     s.nItems += ntiles * g.nchunks;
     s.nSlots += ldq;
     s.usedPart += ntiles * g.nchunks * g.tile;
-    s.usedX += static_cast<size_t>(g.m->D) * ldq;
+    s.usedX += static_cast<size_t>(g.m->D) * ldq * (g.m->lo ? 2 : 1);
     s.usedOut += g.q;
   }
   return !taken.empty();
@@ -957,15 +1056,16 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
 void Server::launch_batch(Slot& s) {
   const size_t esz = cfg_.precision == Precision::FP64 ? 8 : 4;
   // each request's rows into its model's block, padding columns zero
+  // (FP32 models with low parts: rows d in [D, 2D) are the low parts)
   for (const Piece& pc : s.pieces) {
     const Request& r = *pc.r;
-    const size_t D = r.X.size() / (r.q * esz);
-    for (size_t d = 0; d < D; ++d)
+    const size_t rows = r.X.size() / (r.q * esz);
+    for (size_t d = 0; d < rows; ++d)
       std::memcpy(s.hX + (pc.xoff + d * pc.ldq + pc.qoff) * esz,
                   r.X.data() + (d * r.q + pc.q0) * esz, pc.nq * esz);
   }
   for (const Slot::Block& b : s.blocks)
-    for (int d = 0; d < b.D; ++d)
+    for (int d = 0; d < b.D * (b.lo ? 2 : 1); ++d)
       std::memset(s.hX + (b.xoff + d * b.ldq + b.q) * esz, 0, (b.ldq - b.q) * esz);
   s.hCounts[0] = static_cast<int>(s.nItems);
   s.hCounts[1] = static_cast<int>(s.nSlots);
@@ -1010,6 +1110,8 @@ void Server::finish_batch(Slot& s) {
   const double t = now_s();
   for (const Piece& pc : s.pieces) {
     Request& r = *pc.r;
+    auto mp = models_.find(r.model);
+    if (mp != models_.end()) mp->second->inflight--;
     if (r.status != REQ_QUEUED && r.status != REQ_RUNNING) continue;  // failed or cancelled
     if (s.inject) {
       fail_request(pc.r, REQ_FAILED_DEVICE, "injected kernel error");
@@ -1076,15 +1178,20 @@ void Server::tune(Model& m) {
   const size_t nq = kTuneQueries * (cfg_.max_requests_per_batch == 1 ? 1 : kTuneBatchCopies);
   const size_t tile = static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D);
   const size_t ntiles = nq / tile;
-  std::vector<char> hx(nq * D * esz);
+  std::vector<char> hx(nq * D * esz * (m.lo ? 2 : 1));
   uint64_t st = 0x5EED0000ull + static_cast<uint64_t>(D);
   for (size_t d = 0; d < D; ++d)
     for (size_t i = 0; i < nq; ++i) {
       const double v = static_cast<double>(splitmix64(st) >> 11) * 0x1.0p-53;
-      if (f64)
+      if (f64) {
         reinterpret_cast<double*>(hx.data())[d * nq + i] = v;
-      else
-        reinterpret_cast<float*>(hx.data())[d * nq + i] = static_cast<float>(v);
+      } else {
+        const float hi = static_cast<float>(v);
+        reinterpret_cast<float*>(hx.data())[d * nq + i] = hi;
+        if (m.lo)
+          reinterpret_cast<float*>(hx.data())[(D + d) * nq + i] =
+              static_cast<float>(v - static_cast<double>(hi));
+      }
     }
   std::vector<int> cands;
   for (int c : kTuneChunks)
@@ -1129,7 +1236,7 @@ void Server::tune(Model& m) {
       const size_t nchunks = (m.Npad + c - 1) / c;
       const size_t nitems = ntiles * nchunks;
       const Span sp = make_span(m.table_index, m.D, m.Npad, tile, static_cast<size_t>(c), nq, nq,
-                                0, 0, 0, 0, 0);
+                                0, 0, 0, 0, 0, m.lo);
       const int counts[4] = {static_cast<int>(nitems), static_cast<int>(nq), 1, 0};
       check_hip(hipMemcpy(dSpans, &sp, sizeof(Span), hipMemcpyHostToDevice), "upload tune span");
       check_hip(hipMemcpy(dCounts, counts, sizeof(counts), hipMemcpyHostToDevice),

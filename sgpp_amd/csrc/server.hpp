@@ -112,9 +112,10 @@ class Server {
   // queries: q x d row-major FP64 in [0,1]^d. deadline_s <= 0: none
   // (seconds after submission). Returns a request id (never blocks).
   uint64_t submit(int model, size_t q, const double* queries, double deadline_s);
-  // status of a request; when done, copies the q results into out (if not null)
-  // and releases the request. wait_s < 0 waits forever, 0 polls.
-  int poll(uint64_t id, double* out, double wait_s, double* latency_s);
+  // status of a request; when done, copies the q results into out (if not
+  // null; out_capacity must be at least q, else input_error and the request
+  // stays) and releases the request. wait_s < 0 waits forever, 0 polls.
+  int poll(uint64_t id, double* out, size_t out_capacity, double wait_s, double* latency_s);
   bool cancel(uint64_t id);
   // makes the next batch report a device error on all its requests
   void inject_kernel_error();

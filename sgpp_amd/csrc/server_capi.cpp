@@ -138,10 +138,11 @@ sgpp_hip_status sgpp_hip_server_submit(sgpp_hip_server* h, int model_id, size_t 
 }
 
 sgpp_hip_status sgpp_hip_server_poll(sgpp_hip_server* h, uint64_t request_id, double* out,
-                                     double wait_s, int* status, double* latency_s) {
+                                     size_t out_capacity, double wait_s, int* status,
+                                     double* latency_s) {
   return guarded([&] {
     if (!h || !status) throw sgpp_hip::input_error("null argument");
-    const int st = h->s->poll(request_id, out, wait_s, latency_s);
+    const int st = h->s->poll(request_id, out, out_capacity, wait_s, latency_s);
     if (st < 0) {
       g_server_error = "unknown request id";
       return SGPP_HIP_ERR_NOT_FOUND;

@@ -26,6 +26,9 @@ struct SolveResult {
   size_t fp32_matvecs = 0;
   double residual_rel = 0.0;
   double seconds = 0.0;
+  // the stopping test was met (FP64/FP32 modes: CG's delta_new <= eps^2
+  // delta_0; mixed: the FP64 residual ||r|| <= eps ||b||), not a budget
+  bool converged = false;
 };
 
 class DeviceRegression {
@@ -41,6 +44,9 @@ class DeviceRegression {
  private:
   void apply(const double* v, double* q, double lambda, Precision p);
   double dot(const double* x, const double* y);
+  void dot_dev(const double* x, const double* y, double* out);
+  const double* read_scalars(const double* s, int n);
+  void one_thread(void (*kernel)(double*));
   void axpy(double a, const double* x, double* y);
   void xpby(const double* x, double b, double* y);
   void sub(const double* x, const double* y, double* z);
@@ -55,7 +61,8 @@ class DeviceRegression {
   double* tempM_ = nullptr;
   void* dotpart_ = nullptr;
   void* hostscal_ = nullptr;
-  double *alpha_, *b_, *r_, *d_, *q_, *c_, *w_;
+  double *alpha_, *b_, *r_, *d_, *q_, *c_, *w_, *corr_, *rhs_;
+  double* scal_ = nullptr;  // CG scalars in device memory
   size_t fp64_matvecs_ = 0, fp32_matvecs_ = 0;
 };
 

@@ -13,7 +13,10 @@
 // inside the inner CG use the FP32-with-FP64-accumulation kernels:
 //   r = b - A64 alpha; repeat { solve A32 c = r to inner_eps (CG, FP64
 //   vectors); alpha += c; r = b - A64 alpha } until ||r|| <= eps ||b||.
-// Dot products use a fixed two-stage reduction (deterministic).
+// The inner solves share one budget of max_iterations CG iterations.
+// Dot products use a fixed two-stage reduction (deterministic). The CG
+// scalars (d.q, the step a, beta) stay on the device; the host reads d.q and
+// delta_new once per iteration, for the stopping tests.
 
 #include <hip/hip_runtime.h>
 
@@ -51,6 +54,31 @@ __global__ void sub_kernel(const double* __restrict__ x, const double* __restric
 __global__ void fill_kernel(double* __restrict__ y, double v, int64_t n) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) y[i] = v;
+}
+// y += sign * (*a) * x, the step read from device memory
+__global__ void axpy_dev_kernel(double sign, const double* __restrict__ a,
+                                const double* __restrict__ x, double* __restrict__ y, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += (sign * a[0]) * x[i];
+}
+// y = x + (*b) * y
+__global__ void xpby_dev_kernel(const double* __restrict__ x, const double* __restrict__ b,
+                                double* __restrict__ y, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i] + b[0] * y[i];
+}
+// CG scalars in device memory
+enum { kDeltaNew = 0, kDq = 1, kStep = 2, kDeltaOld = 3, kBeta = 4, kNumScalars = 8 };
+// step a = delta_new / (d.q); 0 when d.q == 0 (the host then stops, and the
+// iteration changes no iterate: ConjugateGradients breaks before its update)
+__global__ void cg_step_kernel(double* __restrict__ s) {
+  if (threadIdx.x == 0) s[kStep] = s[kDq] != 0.0 ? s[kDeltaNew] / s[kDq] : 0.0;
+}
+__global__ void cg_keep_delta_kernel(double* __restrict__ s) {
+  if (threadIdx.x == 0) s[kDeltaOld] = s[kDeltaNew];
+}
+__global__ void cg_beta_kernel(double* __restrict__ s) {
+  if (threadIdx.x == 0) s[kBeta] = s[kDeltaNew] / s[kDeltaOld];
 }
 
 // stage 1: block b sums the contiguous chunk [b*chunk, (b+1)*chunk)
@@ -92,11 +120,12 @@ DeviceRegression::DeviceRegression(DeviceMultiEval& op) : op_(op) {
   N_ = op.num_grid();
   M_ = op.num_data();
   if (N_ == 0) throw device_error("solver: grid not set");
-  const size_t nvec = 7;
+  const size_t nvec = 9;
   check_hip(hipMalloc(&vec_, nvec * N_ * sizeof(double)), "hipMalloc(solver vectors)");
   check_hip(hipMalloc(&tempM_, M_ * sizeof(double)), "hipMalloc(solver temp)");
-  check_hip(hipMalloc(&dotpart_, (kDotBlocks + 1) * sizeof(double)), "hipMalloc(dot)");
-  check_hip(hipHostMalloc(&hostscal_, sizeof(double)), "hipHostMalloc");
+  check_hip(hipMalloc(&dotpart_, (kDotBlocks + 1 + kNumScalars) * sizeof(double)),
+            "hipMalloc(dot)");
+  check_hip(hipHostMalloc(&hostscal_, kNumScalars * sizeof(double)), "hipHostMalloc");
   alpha_ = static_cast<double*>(vec_);
   b_ = alpha_ + N_;
   r_ = b_ + N_;
@@ -104,6 +133,9 @@ DeviceRegression::DeviceRegression(DeviceMultiEval& op) : op_(op) {
   q_ = d_ + N_;
   c_ = q_ + N_;
   w_ = c_ + N_;
+  corr_ = w_ + N_;
+  rhs_ = corr_ + N_;
+  scal_ = static_cast<double*>(dotpart_) + kDotBlocks + 1;
 }
 
 DeviceRegression::~DeviceRegression() {
@@ -127,21 +159,36 @@ void DeviceRegression::apply(const double* v, double* q, double lambda, Precisio
     ++fp32_matvecs_;
 }
 
-double DeviceRegression::dot(const double* x, const double* y) {
+// x . y into device memory (out), fixed-order two-stage reduction
+void DeviceRegression::dot_dev(const double* x, const double* y, double* out) {
   const int64_t n = static_cast<int64_t>(N_);
   const int64_t chunk = (n + kDotBlocks - 1) / kDotBlocks;
   double* part = static_cast<double*>(dotpart_);
   clear_launch_error();
   hipLaunchKernelGGL(dot_stage1, dim3(kDotBlocks), dim3(kRB), 0, op_.stream(), x, y, n, chunk,
                      part);
-  hipLaunchKernelGGL(dot_stage2, dim3(1), dim3(kRB), 0, op_.stream(), part, kDotBlocks,
-                     part + kDotBlocks);
+  hipLaunchKernelGGL(dot_stage2, dim3(1), dim3(kRB), 0, op_.stream(), part, kDotBlocks, out);
   check_hip(hipGetLastError(), "dot");
-  check_hip(hipMemcpyAsync(hostscal_, part + kDotBlocks, sizeof(double), hipMemcpyDeviceToHost,
-                           op_.stream()),
-            "dot copy");
-  check_hip(hipStreamSynchronize(op_.stream()), "dot sync");
-  return *static_cast<double*>(hostscal_);
+}
+
+// copies n device scalars to the host and waits for them (one synchronisation)
+const double* DeviceRegression::read_scalars(const double* s, int n) {
+  check_hip(hipMemcpyAsync(hostscal_, s, n * sizeof(double), hipMemcpyDeviceToHost, op_.stream()),
+            "scalar copy");
+  check_hip(hipStreamSynchronize(op_.stream()), "scalar sync");
+  return static_cast<const double*>(hostscal_);
+}
+
+double DeviceRegression::dot(const double* x, const double* y) {
+  double* out = static_cast<double*>(dotpart_) + kDotBlocks;
+  dot_dev(x, y, out);
+  return read_scalars(out, 1)[0];
+}
+
+void DeviceRegression::one_thread(void (*kernel)(double*)) {
+  clear_launch_error();
+  hipLaunchKernelGGL(kernel, dim3(1), dim3(64), 0, op_.stream(), scal_);
+  check_hip(hipGetLastError(), "cg scalars");
 }
 
 void DeviceRegression::axpy(double a, const double* x, double* y) {
@@ -196,26 +243,43 @@ size_t DeviceRegression::cg(double* x, const double* rhs, double lambda, Precisi
     sub(r_, w_, r_);
   }
   copy(r_, d_);
-  double delta_new = dot(r_, r_);
+  dot_dev(r_, r_, scal_ + kDeltaNew);
+  double delta_new = read_scalars(scal_ + kDeltaNew, 1)[0];
   if (!reuse) delta0 = delta_new * eps2;
   if (start_delta) *start_delta = reuse ? delta0 / eps2 : delta_new;
   size_t it = 0;
+  const unsigned nb = blocks_for(N_);
+  const int64_t n = static_cast<int64_t>(N_);
   while (it < max_it && delta_new > delta0) {
     apply(d_, q_, lambda, p);
-    const double dq = dot(d_, q_);
-    if (dq == 0.0) break;
-    const double a = delta_new / dq;
-    axpy(a, d_, x);
+    dot_dev(d_, q_, scal_ + kDq);
+    one_thread(cg_step_kernel);
+    clear_launch_error();
+    hipLaunchKernelGGL(axpy_dev_kernel, dim3(nb), dim3(kRB), 0, op_.stream(), 1.0,
+                       scal_ + kStep, d_, x, n);
+    check_hip(hipGetLastError(), "axpy");
     if ((it % 50) == 0 && it > 0) {
       apply(x, w_, lambda, p);
       sub(rhs, w_, r_);
     } else {
-      axpy(-a, q_, r_);
+      clear_launch_error();
+      hipLaunchKernelGGL(axpy_dev_kernel, dim3(nb), dim3(kRB), 0, op_.stream(), -1.0,
+                         scal_ + kStep, q_, r_, n);
+      check_hip(hipGetLastError(), "axpy");
     }
-    const double delta_old = delta_new;
-    delta_new = dot(r_, r_);
-    const double beta = delta_new / delta_old;
-    xpby(r_, beta, d_);
+    one_thread(cg_keep_delta_kernel);
+    dot_dev(r_, r_, scal_ + kDeltaNew);
+    one_thread(cg_beta_kernel);
+    clear_launch_error();
+    hipLaunchKernelGGL(xpby_dev_kernel, dim3(nb), dim3(kRB), 0, op_.stream(), r_,
+                       scal_ + kBeta, d_, n);
+    check_hip(hipGetLastError(), "xpby");
+    // d.q and delta_new (kDeltaNew = 0, kDq = 1): one synchronisation per
+    // iteration. d.q == 0 stops before the count (the reference's break);
+    // the step was 0, so x is unchanged.
+    const double* h = read_scalars(scal_, 2);
+    if (h[kDq] == 0.0) break;
+    delta_new = h[kDeltaNew];
     ++it;
   }
   if (final_delta) *final_delta = delta_new;
@@ -245,6 +309,8 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
     double fd = 0.0, sd = 0.0;
     res.iterations = cg(alpha_, b_, o.lambda, bp, o.max_iterations, o.epsilon, o.reuse, &fd,
                         &sd, false);
+    // CG's own test (ConjugateGradients.cpp: delta_new <= eps^2 delta_0)
+    res.converged = fd <= o.epsilon * o.epsilon * sd;
   } else {
     // r = b - A64 alpha (alpha = 0 or reused)
     // the outer residual lives in c_, which cg() does not touch
@@ -256,30 +322,22 @@ SolveResult DeviceRegression::solve(const double* y_host, double* alpha_host,
       copy(b_, c_);
     }
     double rnorm = std::sqrt(dot(c_, c_));
-    double* corr = nullptr;
-    check_hip(hipMalloc(&corr, N_ * sizeof(double)), "hipMalloc(correction)");
-    double* rhs = nullptr;
-    check_hip(hipMalloc(&rhs, N_ * sizeof(double)), "hipMalloc(rhs)");
-    try {
-      size_t steps = 0;
-      while (rnorm > o.epsilon * bnorm && steps < o.max_refinements) {
-        copy(c_, rhs);
-        res.iterations += cg(corr, rhs, o.lambda, Precision::FP32, o.max_iterations,
-                             o.inner_epsilon, false, nullptr, nullptr, true);
-        axpy(1.0, corr, alpha_);
-        apply(alpha_, q_, o.lambda, Precision::FP64);
-        sub(b_, q_, c_);
-        rnorm = std::sqrt(dot(c_, c_));
-        ++steps;
-      }
-      res.refinements = steps;
-    } catch (...) {
-      (void)hipFree(corr);
-      (void)hipFree(rhs);
-      throw;
+    size_t steps = 0;
+    // the inner solves share the caller's CG budget (max_iterations in all)
+    while (rnorm > o.epsilon * bnorm && steps < o.max_refinements &&
+           res.iterations < o.max_iterations) {
+      copy(c_, rhs_);
+      res.iterations += cg(corr_, rhs_, o.lambda, Precision::FP32,
+                           o.max_iterations - res.iterations, o.inner_epsilon, false, nullptr,
+                           nullptr, true);
+      axpy(1.0, corr_, alpha_);
+      apply(alpha_, q_, o.lambda, Precision::FP64);
+      sub(b_, q_, c_);
+      rnorm = std::sqrt(dot(c_, c_));
+      ++steps;
     }
-    check_hip(hipFree(corr), "hipFree");
-    check_hip(hipFree(rhs), "hipFree");
+    res.refinements = steps;
+    res.converged = rnorm <= o.epsilon * bnorm;
   }
   // final residual ||b - A alpha|| / ||b||
   apply(alpha_, q_, o.lambda, rp);

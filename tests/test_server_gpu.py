@@ -2,8 +2,12 @@
 and per-request failure handling (budget, deadline, cancel, injected device
 error) that never stops the loop.
 
-Tolerances: FP32-with-FP64-accumulation server 1e-5 relative, FP64 server
-1e-12 relative (||got - ref||_2 / ||ref||_2 per request)."""
+Query points are drawn in FP64 and submitted as they are; the FP32 server
+rounds them itself. Tolerances: FP32-with-FP64-accumulation server 1e-5
+relative, FP64 server 1e-12 relative (||got - ref||_2 / ||ref||_2 per
+request)."""
+import time
+
 import numpy as np
 import pytest
 
@@ -54,8 +58,6 @@ def test_results_match_oracle(gpu, prec, tol):
     for k, q in enumerate([1, 7, 300, 1025, 5000]):
         for m, (b, lv, ix, a) in zip(ids, models):
             X = workloads.uniform_points(60 + k, q, lv.shape[1])
-            if prec == 1:
-                X = X.astype(np.float32).astype(np.float64)
             rid, n = srv.submit(m, X)
             reqs.append((rid, n, X, b, lv, ix, a))
     for rid, n, X, b, lv, ix, a in reqs:
@@ -120,8 +122,6 @@ def test_large_requests_are_split(gpu, prec, tol):
     for k, q in enumerate([20000, 9000]):
         for j, (b, lv, ix, a) in enumerate(models):
             X = workloads.uniform_points(110 + 4 * k + j, q, lv.shape[1])
-            if prec == 1:
-                X = X.astype(np.float32).astype(np.float64)
             small = split.submit(ids_s[j], X[:q // 50])  # traffic between the pieces
             reqs.append((split.submit(ids_s[j], X), whole.submit(ids_w[j], X), small, X, j))
     n_split = 0
@@ -159,8 +159,7 @@ def test_autotune_shapes_and_cache(gpu, tmp_path):
         shapes[(b, lv.shape[1])] = sh
     for k, q in enumerate([3, 700, 4000]):
         for m, (b, lv, ix, a) in zip(ids, models):
-            X = workloads.uniform_points(90 + k, q, lv.shape[1]).astype(np.float32)
-            X = X.astype(np.float64)
+            X = workloads.uniform_points(90 + k, q, lv.shape[1])
             rid, n = srv.submit(m, X)
             st, r, _ = srv.poll(rid, n)
             assert st == capi.REQ_DONE
@@ -247,10 +246,176 @@ def test_shared_subtile_dimensions(gpu, prec, tol):
         alpha = workloads.normal(workloads.stream(120 + i, 0), g.size)
         mid = srv.admit_model(basis, lv, ix, alpha)
         X = workloads.uniform_points(130 + i, 3000, d)
-        if prec == 1:
-            X = X.astype(np.float32).astype(np.float64)
         rid, n = srv.submit(mid, X)
         st, r, _ = srv.poll(rid, n)
         assert st == capi.REQ_DONE
         assert rel(r, oracle.mult(basis, X, lv, ix, alpha)) <= tol, (basis, d)
+    srv.close()
+
+
+def test_fp64_queries_at_high_levels(gpu):
+    """Models whose levels make FP32 rounding of the queries matter (up to
+    2^(l-25) per factor: level 12 gives 1.2e-4). FP64-drawn queries through
+    the FP32 server stay within 1e-5 of the oracle on the same FP64 points:
+    the server adds each coordinate's low part where L >= 32."""
+    capi = gpu
+    srv = small_server(capi, 1)
+    specs = [(capi.BASIS_LINEAR, 2, 12, 0), (capi.BASIS_MODLINEAR, 3, 10, 0),
+             (capi.BASIS_LINEAR, 4, 8, 12), (capi.BASIS_MODLINEAR, 6, 6, 40),
+             (capi.BASIS_LINEAR, 12, 4, 60)]
+    worst = 0.0
+    for i, (basis, d, level, refine) in enumerate(specs):
+        g = capi.Grid(basis, d).regular(level)
+        for step in range(3 if refine else 0):
+            g.refine_surplus(1.0 + workloads.uniform(workloads.stream(140 + i, step), g.size),
+                             refine)
+        lv, ix = g.level_index()
+        assert lv.max() >= 5
+        alpha = workloads.normal(workloads.stream(150 + i, 0), g.size)
+        mid = srv.admit_model(basis, lv, ix, alpha)
+        for k, q in enumerate((1, 333, 2500)):
+            X = workloads.uniform_points(160 + 4 * i + k, q, d)
+            rid, n = srv.submit(mid, X)
+            st, r, _ = srv.poll(rid, n)
+            assert st == capi.REQ_DONE
+            err = rel(r, oracle.mult(basis, X, lv, ix, alpha))
+            worst = max(worst, err)
+            assert err <= 1e-5, (basis, d, level, q, err)
+    print("worst request error of the FP32 server on FP64 queries: %.2e" % worst)
+    srv.close()
+
+
+def test_served_64_model_poisson_stream(gpu):
+    """The served workload: 64 models (d 2..20, Linear/ModLinear, regular or
+    adaptive, 1e3..5e5 points) and 0.5 s of its Poisson request stream at
+    2000 req/s (q 1..1e5, FP64-drawn queries), on an FP32 and an FP64 server.
+
+    Tolerance. FP32 evaluation of a sum r = sum_j a_j phi_j(x) has a forward
+    error bounded by c * d * 2^-24 * sum_j |a_j| phi_j(x) (each term carries
+    the rounding of its d factors), so for every request
+        ||r32 - r64|| <= 1e-5 * ||B |alpha| ||      (B|alpha| from the FP64 server),
+    and, for requests whose sum does not cancel (kappa = ||B|alpha||| / ||r64||
+    <= 10), the plain relative error ||r32 - r64|| / ||r64|| <= 1e-5. Requests
+    of one or a few queries whose terms cancel (kappa up to ~10^3) can exceed
+    1e-5 in the plain relative error at the same forward error; the worst
+    are printed. Sampled requests are checked against the oracle (first 64
+    queries of every 32nd request: FP64 1e-12, FP32 by the same two rules)."""
+    capi = gpu
+    c = workloads.config_serving(duration=0.5)
+    dims = [m["dim"] for m in c["models"]]
+    servers = {1: capi.Server(precision=1), 0: capi.Server(precision=0)}
+    ids = {p: [s.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
+               for m in c["models"]] for p, s in servers.items()}
+    ids_abs = [servers[0].admit_model(m["basis"], m["level"], m["index"], np.abs(m["alpha"]))
+               for m in c["models"]]
+    reqs = []
+    for k in range(len(c["q"])):
+        m, q, off = int(c["model"][k]), int(c["q"][k]), int(c["offset"][k])
+        X = c["pools"][dims[m]][off:off + q]
+        rid = {p: s.submit(ids[p][m], X) for p, s in servers.items()}
+        rid["abs"] = servers[0].submit(ids_abs[m], X)
+        reqs.append((k, m, X, rid))
+    worst_fwd = worst_plain = 0.0
+    above = []
+    checked = 0
+    for k, m, X, rid in reqs:
+        out = {}
+        for p, s in servers.items():
+            st, r, _ = s.poll(*rid[p])
+            assert st == capi.REQ_DONE, (k, p, capi.REQ_NAMES[st])
+            out[p] = r
+        st, rabs, _ = servers[0].poll(*rid["abs"])
+        assert st == capi.REQ_DONE
+        dn = np.linalg.norm(out[1] - out[0])
+        fwd = dn / np.linalg.norm(rabs)
+        plain = dn / max(np.linalg.norm(out[0]), 1e-300)
+        kappa = np.linalg.norm(rabs) / max(np.linalg.norm(out[0]), 1e-300)
+        worst_fwd = max(worst_fwd, fwd)
+        worst_plain = max(worst_plain, plain)
+        assert fwd <= 1e-5, (k, m, X.shape, fwd)
+        if kappa <= 10.0:
+            assert plain <= 1e-5, (k, m, X.shape, plain, kappa)
+        if plain > 1e-5:
+            above.append((k, X.shape[0], plain, kappa))
+        if k % 32 == 0:
+            mm = c["models"][m]
+            rows = X[:64]
+            ref = oracle.mult(mm["basis"], rows, mm["level"], mm["index"], mm["alpha"])
+            refa = oracle.mult(mm["basis"], rows, mm["level"], mm["index"], np.abs(mm["alpha"]))
+            assert rel(out[0][:64], ref) <= 1e-12, (k, m)
+            assert np.linalg.norm(out[1][:64] - ref) <= 1e-5 * np.linalg.norm(refa), (k, m)
+            checked += 1
+    print("%d requests: worst forward error %.2e of ||B|alpha|||, worst plain relative error "
+          "%.2e; above 1e-5 plain (request, q, error, kappa): %s; %d checked against the oracle"
+          % (len(reqs), worst_fwd, worst_plain, above, checked))
+    assert len(reqs) > 800 and checked >= 25
+    for s in servers.values():
+        assert s.stats()["requests_failed"] == 0
+        s.close()
+
+
+def test_cancelled_split_request_blocks_eviction_until_its_pieces_retire(gpu):
+    """A request split over many batches is cancelled while pieces of it may
+    sit in formed or running batches. Eviction of its model is refused until
+    those pieces have retired (the kernels would read the model's freed
+    arrays otherwise); after a drain it succeeds and the server keeps
+    serving other models correctly."""
+    capi = gpu
+    models = small_models(capi)
+    srv = small_server(capi, 1, max_batch_queries=1 << 12, max_items=64,
+                       max_partial_doubles=1 << 16)
+    ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    d2 = models[2][1].shape[1]
+    refused = 0
+    for rep in range(4):
+        victim = srv.admit_model(*models[2])
+        rid, n = srv.submit(victim, workloads.uniform_points(170 + rep, 60000, d2))
+        time.sleep(0.0005 * rep)
+        cancelled = srv.cancel(rid)
+        try:
+            srv.evict_model(victim)
+            evicted = True
+        except capi.SGppHIPError as e:
+            assert "queued or running" in str(e)
+            evicted = False
+            refused += 1
+        st = srv.poll(rid, n)[0]
+        assert st == (capi.REQ_CANCELLED if cancelled else capi.REQ_DONE)
+        srv.drain()
+        if not evicted:
+            srv.evict_model(victim)
+    b, lv, ix, a = models[1]
+    X = workloads.uniform_points(180, 500, lv.shape[1])
+    st, r, _ = srv.poll(*srv.submit(ids[1], X))
+    assert st == capi.REQ_DONE and rel(r, oracle.mult(b, X, lv, ix, a)) <= 1e-5
+    print("eviction refused %d of 4 times" % refused)
+    srv.close()
+
+
+def test_poll_refuses_a_short_output_buffer(gpu):
+    """sgpp_hip_server_poll copies a finished request's results only into a
+    buffer that holds all of them; a short one is refused and the request
+    stays available. submit refuses query arrays of the wrong width."""
+    import ctypes
+
+    capi = gpu
+    models = small_models(capi)
+    srv = small_server(capi, 1)
+    b, lv, ix, a = models[1]
+    mid = srv.admit_model(b, lv, ix, a)
+    X = workloads.uniform_points(190, 100, lv.shape[1])
+    rid, n = srv.submit(mid, X)
+    srv.drain()
+    out = np.zeros(n - 1)
+    st = ctypes.c_int(-1)
+    lat = ctypes.c_double(0)
+    code = capi.F["sgpp_hip_server_poll"](srv.h, rid, out.ctypes.data_as(capi._dp), n - 1, -1.0,
+                                          ctypes.byref(st), ctypes.byref(lat))
+    assert code == 1 and np.all(out == 0.0)
+    st, r, _ = srv.poll(rid, n)
+    assert st == capi.REQ_DONE and rel(r, oracle.mult(b, X, lv, ix, a)) <= 1e-5
+    with pytest.raises(ValueError):
+        srv.submit(mid, X[:, :-1])
+    with pytest.raises(ValueError):
+        srv.submit(mid, X[0])
     srv.close()

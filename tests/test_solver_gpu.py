@@ -139,3 +139,23 @@ def test_fp32_products_cg(gpu):
     # an FP32-only operator cannot run the FP64 solve
     with pytest.raises(capi.SGppHIPError):
         op32.solve(y, lam, mode=capi.SOLVE_FP64)
+
+
+def test_mixed_mode_keeps_the_callers_iteration_budget(gpu):
+    """max_iterations bounds all inner CG iterations of the mixed solve
+    together (the budget a caller such as LearnerBase::train gives the
+    reference's CG), and a solve that spends its budget before reaching the
+    tolerance reports that it did not converge; with a full budget it does."""
+    capi = gpu
+    g, X, y = small_regression(capi)
+    lv, ix = g.level_index()
+    op = capi.MultiEval(0, X, precision=1, keep_fp64=True).prepare(lv, ix)
+    for budget in (1, 5, 12):
+        a, info = op.solve(y, 1e-4, mode=capi.SOLVE_MIXED, max_iterations=budget, epsilon=1e-12,
+                           inner_epsilon=1e-3)
+        assert info["iterations"] <= budget, (budget, info)
+        assert not info["converged"] and info["residual_rel"] > 1e-12
+    a, info = op.solve(y, 1e-4, mode=capi.SOLVE_MIXED, max_iterations=2000, epsilon=1e-12)
+    assert info["converged"] and info["residual_rel"] <= 1e-12 and info["iterations"] <= 2000
+    a64, info64 = op.solve(y, 1e-4, mode=capi.SOLVE_FP64, max_iterations=3, epsilon=1e-12)
+    assert info64["iterations"] == 3 and not info64["converged"]
