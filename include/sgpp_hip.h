@@ -260,7 +260,8 @@ sgpp_hip_status sgpp_hip_server_evict_model(sgpp_hip_server* s, int model_id);
 sgpp_hip_status sgpp_hip_server_memory_used(sgpp_hip_server* s, size_t* bytes);
 /* queries: q x d row-major in [0,1]^d (d = the model's dimension); deadline_s
  * <= 0 means none. An FP32 server carries each coordinate as fl32(x) plus,
- * for models with levels where the rounding matters, fl32(x - fl32(x)). */
+ * for models with levels where the rounding matters (L = 2^l >= 64),
+ * fl32(x - fl32(x)). */
 sgpp_hip_status sgpp_hip_server_submit(sgpp_hip_server* s, int model_id, size_t q,
                                        const double* queries, double deadline_s,
                                        uint64_t* request_id);

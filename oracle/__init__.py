@@ -329,3 +329,60 @@ class RefOp:
         _rcheck(ref().ref_cg_solve(self.h, self.M, n, _d(yy), lam, max_it, eps, _d(a),
                                    ctypes.byref(it), ctypes.byref(res)))
         return a, it.value, res.value
+
+
+# ----------------------------------------------------- reference learner chain
+
+def learner_check_path():
+    """_ref/learner_check (the reference's LearnerLeastSquaresIdentity and
+    ModelFittingLeastSquares, compiled from its sources, with the HIP adapter
+    behind its factory), or None where it was not built."""
+    p = os.path.join(HERE, "_ref", "learner_check")
+    return p if os.path.exists(p) else None
+
+
+def run_learner_check(mode, op, basis, X, y, level, lam, Xt=None, refine=(1e-10, 1000),
+                      final=None, num_refinements=1, threshold=0.0, num_points=5,
+                      test_acc=False, timeout=600):
+    """Runs the reference's own learner (mode "learner": LearnerLeastSquaresIdentity::train,
+    predict, getAccuracy) or fitter (mode "fitter": ModelFittingLeastSquares fit, adapt,
+    evaluate) in a child process, with the operator op: "streaming" (StreamingDefault),
+    "default", or "hip64" / "hip32" (the HIP operator through the SG++-side adapter).
+    Returns {stage: (level, index, alpha, predictions)} and the info values."""
+    import subprocess
+    import tempfile
+    exe = learner_check_path()
+    if exe is None:
+        raise RuntimeError("oracle/_ref/learner_check is not built (make -C oracle ref)")
+    X, y = _f64(X), _f64(y)
+    M, d = X.shape
+    Xt = X[:1] if Xt is None else _f64(Xt)
+    final = refine if final is None else final
+    with tempfile.TemporaryDirectory() as tmp:
+        X.tofile(os.path.join(tmp, "X.bin"))
+        y.tofile(os.path.join(tmp, "y.bin"))
+        Xt.tofile(os.path.join(tmp, "Xt.bin"))
+        args = [exe, mode, op, tmp, "basis=%d" % basis, "dim=%d" % d, "level=%d" % level,
+                "M=%d" % M, "Mt=%d" % Xt.shape[0], "lambda=%r" % lam,
+                "eps_refine=%r" % refine[0], "it_refine=%d" % refine[1],
+                "eps_final=%r" % final[0], "it_final=%d" % final[1],
+                "refinements=%d" % num_refinements, "points=%d" % num_points,
+                "threshold=%r" % threshold, "test_acc=%d" % int(test_acc)]
+        r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
+        if r.returncode != 0:
+            raise RuntimeError("learner_check failed (%d): %s" % (r.returncode, r.stderr.strip()))
+        info = {}
+        with open(os.path.join(tmp, "info.txt")) as f:
+            for line in f:
+                k, v = line.split()
+                info[k] = float(v)
+        out = {}
+        for stage in ("train", "fit", "adapt"):
+            p = os.path.join(tmp, stage + "_alpha.bin")
+            if not os.path.exists(p):
+                continue
+            lv = np.fromfile(os.path.join(tmp, stage + "_level.bin"), dtype=np.uint32)
+            ix = np.fromfile(os.path.join(tmp, stage + "_index.bin"), dtype=np.uint32)
+            out[stage] = (lv.reshape(-1, d), ix.reshape(-1, d), np.fromfile(p),
+                          np.fromfile(os.path.join(tmp, stage + "_pred.bin")))
+    return out, info

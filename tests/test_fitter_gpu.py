@@ -33,6 +33,33 @@ def dataset(seed=41, M=3000, d=3):
     return X, y
 
 
+@pytest.mark.skipif(oracle.learner_check_path() is None,
+                    reason="reference learner build not available")
+@pytest.mark.parametrize("basis", [0, 1])
+def test_fit_and_adapt_match_reference_fitter(gpu, basis):
+    """fit and one adapt against the reference's own ModelFittingLeastSquares
+    (ModelFittingLeastSquares.cpp:30-142, compiled from its sources into
+    oracle/_ref/learner_check, StreamingDefault operator): grids identical,
+    alpha and evaluate within 1e-8."""
+    capi = gpu
+    X, y = dataset(seed=47)
+    Xt, _ = dataset(seed=48, M=300)
+    cg = (1e-12, 1000)
+    ref, _ = oracle.run_learner_check("fitter", "streaming", basis, X, y, 3, 1e-4, Xt=Xt,
+                                      refine=cg, final=cg, num_refinements=1, num_points=6)
+    f = capi.Fitter(basis, level=3, num_refinements=1, num_points=6, refine=cg, final=cg,
+                    lam=1e-4)
+    f.fit(X, y)
+    for stage in ("fit", "adapt"):
+        if stage == "adapt":
+            assert f.adapt()
+        rlv, rix, ra, rpred = ref[stage]
+        lv, ix = f.level_index()
+        assert np.array_equal(lv, rlv) and np.array_equal(ix, rix), stage
+        assert rel(f.alpha(), ra) <= 1e-8, stage
+        assert rel(f.evaluate(Xt), rpred) <= 1e-8, stage
+
+
 @ref_only
 @pytest.mark.parametrize("basis", [0, 1])
 def test_fit_and_adapt_match_reference_steps(gpu, basis):

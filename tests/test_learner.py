@@ -37,3 +37,25 @@ def test_counts_match_reference(basis, iterations, reuse, solver):
     theirs = rg.performance(12345, iterations, solver=solver, reuse_alpha=reuse)
     assert mine[0] == pytest.approx(theirs[0], rel=1e-14)
     assert mine[1] == pytest.approx(theirs[1], rel=1e-14)
+
+
+@pytest.mark.skipif(oracle.learner_check_path() is None or oracle.ref() is None,
+                    reason="reference builds not available")
+@pytest.mark.parametrize("basis", [0, 1])
+def test_restated_learner_loop_matches_reference_learner(basis):
+    """oracle.learner_train (LearnerBase::train's steps on reference parts,
+    the per-pass oracle of the GPU learner tests) against the reference's own
+    LearnerLeastSquaresIdentity::train: grids identical, alpha within 1e-8."""
+    import numpy as np
+
+    from sgpp_amd import workloads
+    X = workloads.uniform_points(36, 2500, 3)
+    y = np.sin(2 * np.pi * X[:, 0]) * X[:, 1] + X[:, 2] ** 2
+    out, _ = oracle.run_learner_check("learner", "streaming", basis, X, y, 3, 1e-4,
+                                      refine=(1e-12, 1000), num_refinements=3, num_points=5)
+    rlv, rix, ra, _ = out["train"]
+    g, a, _ = oracle.learner_train(basis, X, y, 3, 1e-4, refine=(1e-12, 1000), num_refinements=3,
+                                   num_points=5)
+    lv, ix = g.level_index()
+    assert np.array_equal(lv, rlv) and np.array_equal(ix, rix)
+    assert np.linalg.norm(a - ra) / np.linalg.norm(ra) <= 1e-8

@@ -1,7 +1,9 @@
-"""Regression learner on the GPU operator against the steps of the
-reference's LearnerBase::train run on reference parts (oracle.learner_train:
-reference Grid, StreamingDefault operator, ConjugateGradients with reused
-coefficients, SurplusRefinementFunctor).
+"""Regression learner on the GPU operator against the reference's own
+LearnerLeastSquaresIdentity::train (LearnerBase.cpp:113-327, compiled from
+its sources into oracle/_ref/learner_check, with its StreamingDefault
+operator), and per refinement pass against the same steps on reference parts
+(oracle.learner_train, itself checked against the real learner in
+tests/test_learner.py).
 
 Tolerances: grids identical point for point after every refinement (the
 refinement picks the same top-k surpluses); alpha and train MSE within 1e-8
@@ -72,6 +74,36 @@ def test_train_matches_reference_loop(gpu, basis):
     assert timing["gbyte"] == pytest.approx(gb, rel=1e-12)
     assert timing["time_complete"] > 0
     assert timing["time_mult_compute"] > 0 and timing["time_mult_trans_compute"] > 0
+
+
+learner_check = pytest.mark.skipif(oracle.learner_check_path() is None,
+                                   reason="reference learner build not available")
+
+
+@learner_check
+@pytest.mark.parametrize("basis", [0, 1])
+def test_train_matches_reference_learner(gpu, basis):
+    """The final grid, alpha, predictions and train MSE of the GPU learner
+    against the reference's LearnerLeastSquaresIdentity on the same data."""
+    capi = gpu
+    X, y = dataset(seed=34)
+    Xt, yt = dataset(seed=35, M=400)
+    cfg = dict(refine=(1e-12, 1000), num_refinements=3, num_points=5)
+    ref, info = oracle.run_learner_check("learner", "streaming", basis, X, y, 3, 1e-4, Xt=Xt,
+                                         **cfg)
+    rlv, rix, ra, rpred = ref["train"]
+    lr = capi.Learner(precision=capi.PREC_FP64)
+    timing = lr.train(X, y, basis, 3, 1e-4, refine=cfg["refine"],
+                      num_refinements=cfg["num_refinements"],
+                      num_refinement_points=cfg["num_points"])
+    lv, ix = lr.level_index()
+    assert np.array_equal(lv, rlv) and np.array_equal(ix, rix)
+    assert rel(lr.alpha(), ra) <= 1e-8
+    assert rel(lr.predict(Xt), rpred) <= 1e-8
+    assert lr.accuracy(X, y) == pytest.approx(info["train_mse"], rel=1e-8)
+    # the reference's operation count depends on its CG iteration counts,
+    # which rounding moves by a few iterations
+    assert timing["gflop"] == pytest.approx(info["gflop"], rel=0.1)
 
 
 @ref_only

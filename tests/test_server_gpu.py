@@ -257,7 +257,7 @@ def test_fp64_queries_at_high_levels(gpu):
     """Models whose levels make FP32 rounding of the queries matter (up to
     2^(l-25) per factor: level 12 gives 1.2e-4). FP64-drawn queries through
     the FP32 server stay within 1e-5 of the oracle on the same FP64 points:
-    the server adds each coordinate's low part where L >= 32."""
+    the server adds each coordinate's low part where L >= 64."""
     capi = gpu
     srv = small_server(capi, 1)
     specs = [(capi.BASIS_LINEAR, 2, 12, 0), (capi.BASIS_MODLINEAR, 3, 10, 0),
