@@ -126,6 +126,18 @@ sgpp_hip_status sgpp_hip_meval_mult_device(sgpp_hip_meval* h, const double* d_al
                                            double* d_result, int precision);
 sgpp_hip_status sgpp_hip_meval_mult_transpose_device(sgpp_hip_meval* h, const double* d_source,
                                                      double* d_result, int precision);
+/* sgpp_hip_meval_mult_device that allocates nothing (no hipMalloc, no
+ * hipFree, no allocating library call). No counterpart in the reference: it
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). The alpha weights and the
+ * FP64 partials live in a fixed 64 MiB device array per GPU, held until the
+ * process ends and shared by every call on that GPU: calls that overlap on
+ * one GPU race. The grid is cut into fewer chunks so that the partials fit;
+ * inputs needing more than 64 MiB at one chunk (N x 4 or 8 bytes plus about
+ * M x 8 bytes) are refused with SGPP_HIP_ERR_INVALID_ARGUMENT. Asynchronous
+ * on the operator's stream, like sgpp_hip_meval_mult_device. */
+sgpp_hip_status sgpp_hip_meval_mult_device_noalloc(sgpp_hip_meval* h, const double* d_alpha,
+                                                   double* d_result, int precision);
 sgpp_hip_status sgpp_hip_meval_synchronize(sgpp_hip_meval* h);
 /* getDuration() (OperationMultipleEval.hpp:120): seconds of the last host call */
 double sgpp_hip_meval_duration(const sgpp_hip_meval* h);

@@ -159,6 +159,7 @@ SIGNATURES = {
     "sgpp_hip_meval_last_shape": (_st, [_p, ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]),
     "sgpp_hip_meval_kernel_timing_enable": (_st, [_p, ctypes.c_int]),
+    "sgpp_hip_meval_mult_device_noalloc": (_st, [_p, _p, _p, ctypes.c_int]),
     "sgpp_hip_meval_count_work": (_st, [_p, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
     "sgpp_hip_meval_kernel_timing": (_st, [_p, ctypes.POINTER(ctypes.c_int), _dp,
                                            ctypes.POINTER(ctypes.c_int), _dp]),

@@ -387,6 +387,15 @@ sgpp_hip_status sgpp_hip_meval_mult_device(sgpp_hip_meval* h, const double* d_al
   });
 }
 
+sgpp_hip_status sgpp_hip_meval_mult_device_noalloc(sgpp_hip_meval* h, const double* d_alpha,
+                                                   double* d_result, int precision) {
+  return guarded([&] {
+    REQUIRE(h && d_alpha && d_result, "null argument");
+    h->dev->mult_device_noalloc(d_alpha, d_result, prec_of(precision));
+    return SGPP_HIP_OK;
+  });
+}
+
 sgpp_hip_status sgpp_hip_meval_mult_transpose_device(sgpp_hip_meval* h, const double* d_source,
                                                      double* d_result, int precision) {
   return guarded([&] {

@@ -91,6 +91,12 @@ class DeviceMultiEval {
   // Device-pointer entry points (inputs/outputs already in HBM, FP64).
   // alpha: N values, result: M values; y: M values, result: N values.
   void mult_device(const double* d_alpha, double* d_result, Precision p);
+  // mult_device without any allocation: the alpha weights and the FP64
+  // partials live in the library's fixed device scratch (kNoAllocScratchBytes
+  // per GPU, shared by every call on that GPU: calls that overlap on one GPU
+  // race). The grid chunks are capped so that the partials fit; inputs whose
+  // weights and one row of partials do not fit are refused (input_error).
+  void mult_device_noalloc(const double* d_alpha, double* d_result, Precision p);
   void mult_transpose_device(const double* d_y, double* d_result, Precision p);
 
   // Host-pointer entry points (copy in, compute, copy out, synchronise).
@@ -134,9 +140,15 @@ class DeviceMultiEval {
   void ensure(void** buf, size_t* cap, size_t bytes);
   void record_start();
   void record_stop();
+  // Caller-held scratch of a mult launch: weights in the compute type, then
+  // the FP64 partials.
+  struct MultScratch {
+    void* base;
+    size_t bytes;
+  };
   template <typename T>
   void run_mult(const T* X, const T* LI, const float* TB, const double* d_alpha, double* d_result,
-                hipEvent_t* tev);
+                hipEvent_t* tev, const MultScratch* ext = nullptr);
   template <typename T>
   void run_multT(const T* X, const T* LI, const float* TB, const double* d_y, double* d_result,
                  hipEvent_t* tev);
@@ -189,6 +201,9 @@ class DeviceMultiEval {
   double last_wall_s_ = 0.0;
   int last_G_ = 0, last_S_ = 0;
 };
+
+// Size of the fixed device scratch of mult_device_noalloc (per GPU).
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
 
 // Padding units (points): data is padded to a multiple of kDataPad by
 // repeating the last stored row, the grid to a multiple of kGridPad with

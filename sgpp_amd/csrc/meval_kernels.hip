@@ -43,6 +43,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <sstream>
 
@@ -315,12 +316,27 @@ void DeviceMultiEval::record_stop() { check_hip(hipEventRecord(ev1_, stream_), "
 
 template <typename T>
 void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const double* d_alpha,
-                               double* d_result, hipEvent_t* tev) {
+                               double* d_result, hipEvent_t* tev, const MultScratch* ext) {
   hipStream_t s = stream_;
-  ensure(&A_, &A_cap_, Npad_ * sizeof(T));
+  // weights, then partials (with ext: in the caller's scratch, 256-byte aligned)
+  const size_t a_bytes = (Npad_ * sizeof(T) + 255) / 256 * 256;
+  void* A = nullptr;
+  if (ext) {
+    if (a_bytes + Mpad_ * sizeof(double) > ext->bytes) {
+      std::ostringstream os;
+      os << "input too large for the fixed scratch: " << N_ << " grid points and " << M_
+         << " data points need " << (a_bytes + Mpad_ * sizeof(double))
+         << " bytes at one grid chunk, the scratch holds " << ext->bytes;
+      throw input_error(os.str());
+    }
+    A = ext->base;
+  } else {
+    ensure(&A_, &A_cap_, Npad_ * sizeof(T));
+    A = A_;
+  }
   clear_launch_error();
   hipLaunchKernelGGL(prep_alpha_kernel<T>, dim3(ceil_div(Npad_, 256)), dim3(256), 0, s, d_alpha,
-                     scale_, static_cast<const int32_t*>(perm_), static_cast<T*>(A_),
+                     scale_, static_cast<const int32_t*>(perm_), static_cast<T*>(A),
                      static_cast<int64_t>(N_), static_cast<int64_t>(Npad_));
   check_hip(hipGetLastError(), "prep_alpha");
   const int K = mult_k_rt<T>(static_cast<int>(D_));
@@ -337,25 +353,32 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
     // partials G x Mpad doubles within the budget
     G = std::min<int64_t>(G, std::max<int64_t>(1, kPartialBudget / (Mpad_ * sizeof(double))));
   }
+  if (ext)  // partials G x Mpad doubles within the scratch
+    G = std::min<int64_t>(G, (ext->bytes - a_bytes) / (Mpad_ * sizeof(double)));
   G = std::max(1, std::min(G, ntiles));
   const int tiles_per = ceil_div(ntiles, G);
   G = ceil_div(ntiles, tiles_per);
   const int chunk = tiles_per * kTG;
   const int jend = ntiles * kTG;
-  ensure(&part_, &part_cap_, static_cast<size_t>(G) * Mpad_ * sizeof(double));
-  double* P = static_cast<double*>(part_);
+  double* P;
+  if (ext) {
+    P = reinterpret_cast<double*>(static_cast<unsigned char*>(ext->base) + a_bytes);
+  } else {
+    ensure(&part_, &part_cap_, static_cast<size_t>(G) * Mpad_ * sizeof(double));
+    P = static_cast<double*>(part_);
+  }
   if (tev) check_hip(hipEventRecord(tev[0], s), "hipEventRecord");
   if (D_ <= static_cast<size_t>(kMaxTemplD)) {
     const bool skip = basis_ == Basis::ModLinear;
     MultFn<T> f = count_ ? count_mult_launcher<T>(static_cast<int>(D_), skip)
                          : DimTable<T>::mult(static_cast<int>(D_), skip);
-    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A_),
+    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A),
       P, chunk, jend, G, static_cast<int64_t>(Mpad_), count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();
     hipLaunchKernelGGL(mult_kernel_generic<T>, grid, dim3(kBlock), 0, s, X,
-                       static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A_), P, chunk, jend,
+                       static_cast<int64_t>(Mpad_), LI, static_cast<const T*>(A), P, chunk, jend,
                        static_cast<int>(D_));
   }
   check_hip(hipGetLastError(), "mult kernel");
@@ -442,6 +465,48 @@ void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Preci
   else
     run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
                     static_cast<const float*>(TB32_), d_alpha, d_result, tp ? tev : nullptr);
+  record_stop();
+}
+
+namespace {
+// Fixed device scratch of mult_device_noalloc: one copy per GPU, held until
+// the process ends. In host code this name is only a stand-in for the device
+// symbol; the address to use is the one hipGetSymbolAddress reports for the
+// current device (device_scratch).
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// this device's address of the scratch array (looked up once per device,
+// with the device current)
+void* device_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) throw input_error("device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    check_hip(hipSetDevice(device), "hipSetDevice");
+    check_hip(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) throw device_error("hipGetSymbolAddress(scratch) returned a null address");
+    g_scratch[device] = ptr;
+  }
+  return g_scratch[device];
+}
+}  // namespace
+
+void DeviceMultiEval::mult_device_noalloc(const double* d_alpha, double* d_result, Precision p) {
+  if (N_ == 0) throw input_error("grid not set (call prepare)");
+  if (!serves(p)) throw input_error("precision not enabled for this instance");
+  const MultScratch sc{device_scratch(device_), kNoAllocScratchBytes};
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  record_start();
+  if (p == Precision::FP64)
+    run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
+                     static_cast<const float*>(TB64_), d_alpha, d_result, nullptr, &sc);
+  else
+    run_mult<float>(static_cast<const float*>(X32_), static_cast<const float*>(LI32_),
+                    static_cast<const float*>(TB32_), d_alpha, d_result, nullptr, &sc);
   record_stop();
 }
 
