@@ -343,12 +343,16 @@ def learner_check_path():
 
 def run_learner_check(mode, op, basis, X, y, level, lam, Xt=None, refine=(1e-10, 1000),
                       final=None, num_refinements=1, threshold=0.0, num_points=5,
-                      test_acc=False, timeout=600):
+                      test_acc=False, regularization=0, weights=None, timeout=600):
     """Runs the reference's own learner (mode "learner": LearnerLeastSquaresIdentity::train,
-    predict, getAccuracy) or fitter (mode "fitter": ModelFittingLeastSquares fit, adapt,
-    evaluate) in a child process, with the operator op: "streaming" (StreamingDefault),
-    "default", or "hip64" / "hip32" (the HIP operator through the SG++-side adapter).
-    Returns {stage: (level, index, alpha, predictions)} and the info values."""
+    predict, getAccuracy), fitter (mode "fitter": ModelFittingLeastSquares fit, adapt,
+    evaluate), RegressionLearner (mode "regression": train, predict, getMSE, with
+    regularization 0 Identity, 1 Laplace, 2 Diagonal) or DMWeightMatrix in CG (mode
+    "weighted", data weights `weights`) in a child process, with the operator op:
+    "streaming" (StreamingDefault), "default", "hip64" / "hip32" (the HIP operator
+    through the SG++-side adapter, selected by the configuration) or "hipdefault" (the
+    base factory returns the adapter). Returns {stage: (level, index, alpha,
+    predictions)} and the info values."""
     import subprocess
     import tempfile
     exe = learner_check_path()
@@ -362,12 +366,15 @@ def run_learner_check(mode, op, basis, X, y, level, lam, Xt=None, refine=(1e-10,
         X.tofile(os.path.join(tmp, "X.bin"))
         y.tofile(os.path.join(tmp, "y.bin"))
         Xt.tofile(os.path.join(tmp, "Xt.bin"))
+        if weights is not None:
+            _f64(weights).tofile(os.path.join(tmp, "w.bin"))
         args = [exe, mode, op, tmp, "basis=%d" % basis, "dim=%d" % d, "level=%d" % level,
                 "M=%d" % M, "Mt=%d" % Xt.shape[0], "lambda=%r" % lam,
                 "eps_refine=%r" % refine[0], "it_refine=%d" % refine[1],
                 "eps_final=%r" % final[0], "it_final=%d" % final[1],
                 "refinements=%d" % num_refinements, "points=%d" % num_points,
-                "threshold=%r" % threshold, "test_acc=%d" % int(test_acc)]
+                "threshold=%r" % threshold, "test_acc=%d" % int(test_acc),
+                "regularization=%d" % regularization]
         r = subprocess.run(args, capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError("learner_check failed (%d): %s" % (r.returncode, r.stderr.strip()))

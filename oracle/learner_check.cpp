@@ -9,7 +9,10 @@
 //   solver::ConjugateGradients, op_factory::createOperationMultipleEval
 //     (DatadrivenOpFactory.cpp:409-552),
 //   ModelFittingLeastSquares::fit / adapt / evaluate
-//     (datamining/modules/fitting/ModelFittingLeastSquares.cpp:30-142).
+//     (datamining/modules/fitting/ModelFittingLeastSquares.cpp:30-142),
+//   RegressionLearner::train / predict / getMSE with DMSystemMatrix
+//     (application/RegressionLearner.cpp:62-128, algorithm/DMSystemMatrix.cpp:16-86),
+//   DMWeightMatrix (algorithm/DMWeightMatrix.cpp:24-60) in ConjugateGradients.
 //
 // It also proves the drop-in: this program defines
 // op_factory::createOperationMultipleEval itself. A definition in the program
@@ -19,14 +22,24 @@
 // libsgpp_hip.so; every other configuration is passed on to the reference's
 // own factory (found with dlsym(RTLD_NEXT)). This stands in for the one branch
 // a maintainer adds to the factory; no reference file is changed.
+// RegressionLearner, DMSystemMatrix and DMWeightMatrix take their operator
+// from the base factory's createOperationMultipleEval(grid, dataset)
+// (base/src/sgpp/base/operation/BaseOpFactory.cpp:595-625), which has no
+// configuration; with op "hipdefault" this program's definition of that
+// function returns the adapter (FP64) for Linear and ModLinear grids, the
+// branch a maintainer adds there to make HIP the default.
 //
-//   learner_check <learner|fitter> <streaming|default|hip64|hip32> <dir> key=value...
+//   learner_check <learner|fitter|regression|weighted>
+//                 <streaming|default|hip64|hip32|hipdefault> <dir> key=value...
 // dir holds X.bin (M x d), y.bin (M), Xt.bin (Mt x d), raw little-endian
 // float64. Keys: basis (0 Linear, 1 ModLinear), dim, level, M, Mt, lambda,
 // eps_refine, it_refine, eps_final, it_final, refinements, points, threshold,
-// test_acc. Outputs in dir: <stage>_level.bin, <stage>_index.bin (uint32,
-// N x d, sequence order), <stage>_alpha.bin, <stage>_pred.bin (float64) for
-// stage "train" (learner) or "fit" and "adapt" (fitter), and info.txt.
+// test_acc, regularization (0 Identity, 1 Laplace, 2 Diagonal; regression
+// mode). weighted mode also reads w.bin (M weights). Outputs in dir:
+// <stage>_level.bin, <stage>_index.bin (uint32, N x d, sequence order),
+// <stage>_alpha.bin, <stage>_pred.bin (float64) for stage "train" (learner,
+// regression: alpha empty, the learner keeps its weights private; weighted),
+// or "fit" and "adapt" (fitter), and info.txt.
 
 #include <dlfcn.h>
 
@@ -34,8 +47,13 @@
 #include <sgpp/base/datatypes/DataVector.hpp>
 #include <sgpp/base/grid/Grid.hpp>
 #include <sgpp/base/tools/OperationConfiguration.hpp>
+#include <sgpp/base/operation/BaseOpFactory.hpp>
 #include <sgpp/datadriven/DatadrivenOpFactory.hpp>
+#include <sgpp/datadriven/algorithm/DMWeightMatrix.hpp>
 #include <sgpp/datadriven/application/LearnerLeastSquaresIdentity.hpp>
+#include <sgpp/datadriven/application/RegressionLearner.hpp>
+#include <sgpp/datadriven/configuration/RegularizationConfiguration.hpp>
+#include <sgpp/solver/sle/ConjugateGradients.hpp>
 #include <sgpp/datadriven/datamining/modules/fitting/FitterConfigurationLeastSquares.hpp>
 #include <sgpp/datadriven/datamining/modules/fitting/ModelFittingLeastSquares.hpp>
 #include <sgpp/datadriven/operation/hash/DatadrivenOperationCommon.hpp>
@@ -59,8 +77,11 @@ using sgpp::base::Grid;
 using sgpp::datadriven::OperationMultipleEvalConfiguration;
 
 namespace {
-int g_hip_ops = 0;    // operators the factory returned from the HIP branch
-int g_other_ops = 0;  // operators it passed on to the reference's factory
+int g_hip_ops = 0;         // operators the factory returned from the HIP branch
+int g_other_ops = 0;       // operators it passed on to the reference's factory
+bool g_base_hip = false;   // op "hipdefault": the base factory returns the adapter
+int g_base_hip_ops = 0;    // operators the base factory returned from the HIP branch
+int g_base_other_ops = 0;  // operators it passed on to the reference's base factory
 }  // namespace
 
 namespace sgpp {
@@ -82,6 +103,24 @@ base::OperationMultipleEval* createOperationMultipleEval(
   if (!next) throw std::runtime_error("the reference's createOperationMultipleEval not found");
   ++g_other_ops;
   return next(grid, dataset, configuration);
+}
+
+base::OperationMultipleEval* createOperationMultipleEval(base::Grid& grid,
+                                                         base::DataMatrix& dataset) {
+  if (g_base_hip && (grid.getType() == base::GridType::Linear ||
+                     grid.getType() == base::GridType::ModLinear)) {
+    ++g_base_hip_ops;
+    datadriven::OperationMultipleEvalConfiguration cfg(
+        datadriven::OperationMultipleEvalType::STREAMING,
+        datadriven::kOperationMultipleEvalSubTypeHIP);
+    return datadriven::createOperationMultiEvalHIP(grid, dataset, cfg);
+  }
+  using Fn = base::OperationMultipleEval* (*)(base::Grid&, base::DataMatrix&);
+  static Fn next = reinterpret_cast<Fn>(dlsym(
+      RTLD_NEXT, "_ZN4sgpp10op_factory27createOperationMultipleEvalERNS_4base4GridERNS1_10DataMatrixE"));
+  if (!next) throw std::runtime_error("the reference's base createOperationMultipleEval not found");
+  ++g_base_other_ops;
+  return next(grid, dataset);
 }
 
 }  // namespace op_factory
@@ -126,7 +165,7 @@ void write_grid(const std::string& dir, const std::string& stage, Grid& grid,
 OperationMultipleEvalConfiguration config_for(const std::string& op) {
   using sgpp::datadriven::OperationMultipleEvalSubType;
   using sgpp::datadriven::OperationMultipleEvalType;
-  if (op == "default") return OperationMultipleEvalConfiguration();
+  if (op == "default" || op == "hipdefault") return OperationMultipleEvalConfiguration();
   if (op == "streaming")
     return OperationMultipleEvalConfiguration(OperationMultipleEvalType::STREAMING,
                                               OperationMultipleEvalSubType::DEFAULT);
@@ -179,6 +218,7 @@ int run(int argc, char** argv) {
   DataMatrix train(X.data(), M, d), test(Xt.data(), Mt, d);
   DataVector classes(y);
   OperationMultipleEvalConfiguration cfg = config_for(op);
+  g_base_hip = op == "hipdefault";
   const auto gtype = basis == 0 ? sgpp::base::GridType::Linear : sgpp::base::GridType::ModLinear;
   std::ofstream info(dir + "/info.txt");
 
@@ -231,11 +271,56 @@ int run(int argc, char** argv) {
     model.evaluate(test, pred);
     write_grid(dir, "adapt", model.getGrid(), model.getSurpluses(), pred);
     info << "refined " << (refined ? 1 : 0) << "\n";
+  } else if (mode == "regression") {
+    sgpp::base::RegularGridConfiguration gc;
+    gc.type_ = gtype;
+    gc.dim_ = d;
+    gc.level_ = level;
+    sgpp::base::AdaptivityConfiguration ac;
+    ac.numRefinements_ = static_cast<size_t>(num("refinements"));
+    ac.refinementThreshold_ = num("threshold");
+    ac.numRefinementPoints_ = static_cast<size_t>(num("points"));
+    sgpp::datadriven::RegularizationConfiguration rc;
+    const int reg = static_cast<int>(num("regularization"));
+    rc.type_ = reg == 0   ? sgpp::datadriven::RegularizationType::Identity
+               : reg == 1 ? sgpp::datadriven::RegularizationType::Laplace
+                          : sgpp::datadriven::RegularizationType::Diagonal;
+    rc.lambda_ = lambda;
+    rc.exponentBase_ = 2.0;
+    sgpp::datadriven::RegressionLearner learner(
+        gc, ac, cg_config(num("eps_refine"), static_cast<size_t>(num("it_refine"))),
+        cg_config(num("eps_final"), static_cast<size_t>(num("it_final"))), rc);
+    learner.train(train, classes);
+    DataVector pred = learner.predict(test);
+    const double mse = learner.getMSE(train, classes);
+    write_grid(dir, "train", learner.getGrid(), DataVector(0), pred);
+    info.precision(17);
+    info << "train_mse " << mse << "\n";
+  } else if (mode == "weighted") {
+    // (B^T W B + lambda C) alpha = B^T W y with C the identity, solved by CG
+    std::unique_ptr<Grid> grid(basis == 0 ? Grid::createLinearGrid(d)
+                                          : Grid::createModLinearGrid(d));
+    grid->getGenerator().regular(level);
+    DataVector w(read_f64(dir + "/w.bin", M));
+    std::unique_ptr<sgpp::base::OperationMatrix> C(sgpp::op_factory::createOperationIdentity(*grid));
+    sgpp::datadriven::DMWeightMatrix A(*grid, train, *C, lambda, w);
+    DataVector b(grid->getSize()), alpha(grid->getSize(), 0.0);
+    A.generateb(classes, b);
+    sgpp::solver::ConjugateGradients cg(static_cast<size_t>(num("it_final")), num("eps_final"));
+    cg.solve(A, alpha, b, false, false, -1.0);
+    DataVector pred(Mt);
+    std::unique_ptr<sgpp::base::OperationMultipleEval> op_t(
+        sgpp::op_factory::createOperationMultipleEval(*grid, test));
+    op_t->mult(alpha, pred);
+    write_grid(dir, "train", *grid, alpha, pred);
+    info << "iterations " << cg.getNumberIterations() << "\n";
   } else {
     throw std::runtime_error("unknown mode " + mode);
   }
   info << "hip_ops " << g_hip_ops << "\n"
-       << "reference_factory_ops " << g_other_ops << "\n";
+       << "reference_factory_ops " << g_other_ops << "\n"
+       << "base_hip_ops " << g_base_hip_ops << "\n"
+       << "base_reference_ops " << g_base_other_ops << "\n";
   return 0;
 }
 

@@ -80,3 +80,43 @@ def test_reference_fitter_on_the_hip_operator(gpu, basis):
         assert np.array_equal(lv, rlv) and np.array_equal(ix, rix), stage
         assert rel(a, ra) <= 1e-8, stage
         assert rel(pred, rpred) <= 1e-8, stage
+
+
+@pytest.mark.parametrize("basis,regularization", [(0, 0), (0, 1), (1, 0), (1, 2)])
+def test_reference_regression_learner_on_the_hip_operator(gpu, basis, regularization):
+    """RegressionLearner::train (RegressionLearner.cpp:62-128) with its
+    DMSystemMatrix (DMSystemMatrix.cpp:16-86; Identity, Laplace or Diagonal
+    regularization operator, which stays the reference's) and ConjugateGradients,
+    all operators from the base factory: DEFAULT against the HIP adapter."""
+    X, y = dataset(57, M=2000)
+    Xt, _ = dataset(58, M=300)
+    kw = dict(Xt=Xt, refine=(1e-12, 1000), num_refinements=2, num_points=4,
+              regularization=regularization)
+    ref, rinfo = oracle.run_learner_check("regression", "default", basis, X, y, 3, 1e-4, **kw)
+    hip, hinfo = oracle.run_learner_check("regression", "hipdefault", basis, X, y, 3, 1e-4,
+                                          **kw)
+    assert hinfo["base_hip_ops"] > 10 and hinfo["base_reference_ops"] == 0
+    assert rinfo["base_hip_ops"] == 0
+    rlv, rix, _, rpred = ref["train"]
+    lv, ix, _, pred = hip["train"]
+    assert np.array_equal(lv, rlv) and np.array_equal(ix, rix)
+    assert rel(pred, rpred) <= 1e-8
+    assert hinfo["train_mse"] == pytest.approx(rinfo["train_mse"], rel=1e-8)
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+def test_reference_weight_matrix_on_the_hip_operator(gpu, basis):
+    """DMWeightMatrix (DMWeightMatrix.cpp:24-60): (B^T W B + lambda I) alpha =
+    B^T W y in the reference's ConjugateGradients, DEFAULT against the HIP
+    adapter."""
+    X, y = dataset(59, M=2500, d=4)
+    Xt, _ = dataset(60, M=200, d=4)
+    w = 0.5 + X[:, 0] * X[:, 1]
+    kw = dict(Xt=Xt, final=(1e-12, 2000), weights=w)
+    ref, rinfo = oracle.run_learner_check("weighted", "default", basis, X, y, 4, 1e-4, **kw)
+    hip, hinfo = oracle.run_learner_check("weighted", "hipdefault", basis, X, y, 4, 1e-4, **kw)
+    assert hinfo["base_hip_ops"] == 2 and hinfo["base_reference_ops"] == 0
+    ra, rpred = ref["train"][2], ref["train"][3]
+    a, pred = hip["train"][2], hip["train"][3]
+    assert rel(a, ra) <= 1e-8
+    assert rel(pred, rpred) <= 1e-8
