@@ -204,10 +204,15 @@ __device__ __forceinline__ void take_hits(unsigned long long& bal, const T* __re
   }
 }
 
-// bit pattern of a value (L, I >= 0 are zero exactly when their bits are zero)
-__device__ __forceinline__ uint32_t bits_of(float v) { return __float_as_uint(v); }
-__device__ __forceinline__ uint64_t bits_of(double v) {
-  return static_cast<uint64_t>(__double_as_longlong(v));
+// the union of the non-constant-dimension masks of the J taken points: lane
+// jj[q] holds point jj[q]'s mask in pm (unused slots repeat slot 0)
+template <int J>
+__device__ __forceinline__ uint32_t group_dims(uint32_t pm, const int (&jj)[J]) {
+  uint32_t m = 0u;
+#pragma unroll
+  for (int q = 0; q < J; ++q)
+    m |= static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(pm), jj[q]));
+  return m;
 }
 
 // An (L, I) pair, loaded as one 8- or 16-byte vector.
@@ -218,15 +223,14 @@ struct alignas(2 * sizeof(T)) LIPair {
 
 // Products p[q][k] = phi_{jj[q]}(x_k) of the J taken points of a tile. The
 // points' (L, I) pairs are read in batches of DB dimensions (wave-uniform
-// addresses: wide scalar loads). With kSkip, a dimension in which every taken
-// point has (L, I) = (0, 0) (a ModLinear level-1 factor, exactly 1) is
-// skipped (Linear grids have no such dimension). The
-// batch's skip mask is computed from all its loaded pairs before any test, so
-// all loads of the batch are issued together, ahead of the branches.
-// Returns the number of dimensions evaluated (wave-uniform).
+// addresses: wide scalar loads). With kSkip, only the dimensions of `live`
+// are evaluated: the union of the taken points' non-constant dimensions (a
+// ModLinear level-1 factor, (L, I) = (0, 0), is exactly 1); Linear grids
+// have no constant dimension. Returns the number of dimensions evaluated
+// (wave-uniform).
 template <int D, int J, int K, typename T, bool kSkip>
 __device__ __forceinline__ int eval_group(const T* __restrict__ tile, const int (&jj)[J],
-                                          const T (&x)[K][D], T (&p)[J][K]) {
+                                          const T (&x)[K][D], T (&p)[J][K], uint32_t live_all) {
   // dimensions per batch: the batch's pairs take at most 48 scalar registers
   constexpr int kWordsPerDim = J * 2 * static_cast<int>(sizeof(T)) / 4;
   constexpr int DB = D < 48 / kWordsPerDim ? D : 48 / kWordsPerDim;
@@ -244,15 +248,7 @@ __device__ __forceinline__ int eval_group(const T* __restrict__ tile, const int 
 #pragma unroll
       for (int dd = 0; dd < DB; ++dd)
         if (d0 + dd < D) li[q][dd] = rows[jj[q] * D + d0 + dd];
-    uint32_t live = kSkip ? 0u : ~0u;
-#pragma unroll
-    for (int dd = 0; dd < DB; ++dd) {
-      if (!kSkip || d0 + dd >= D) continue;
-      auto any = bits_of(li[0][dd].L) | bits_of(li[0][dd].I);
-#pragma unroll
-      for (int q = 1; q < J; ++q) any |= bits_of(li[q][dd].L) | bits_of(li[q][dd].I);
-      live |= (any != 0 ? 1u : 0u) << dd;
-    }
+    const uint32_t live = kSkip ? (live_all >> d0) : ~0u;
     evaluated += __builtin_popcount(live & ((1u << (D - d0 < DB ? D - d0 : DB)) - 1u));
 #pragma unroll
     for (int dd = 0; dd < DB; ++dd) {

@@ -68,12 +68,13 @@ std::vector<int32_t> morton_order(const double* pts, size_t n, size_t d);
 std::vector<int32_t> grid_order(const std::vector<double>& li, const std::vector<uint32_t>& pmask,
                                 size_t n, size_t d);
 
-// Per 64-point tile and dimension, the union of the points' supports
-// [(I - 1) / L, (I + 1) / L] (the whole line for L = 0), rounded outwards to
-// T. li: point-major (L, I) pairs of npad points (npad a multiple of 64).
+// Per 64-point tile and dimension, the union of the supports of the tile's
+// points below n, [(I - 1) / L, (I + 1) / L] (the whole line for L = 0),
+// rounded outwards to T (an empty box when the tile holds none). li:
+// point-major (L, I) pairs of npad points (npad a multiple of 64).
 // Returns [npad/64][d][2].
 template <typename T>
-std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d);
+std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t n, size_t d);
 
 class DeviceMultiEval {
  public:
@@ -158,6 +159,7 @@ class DeviceMultiEval {
   // of the support centres; perm_[stored position] = caller's index.
   void* dperm_ = nullptr;
   void* perm_ = nullptr;   size_t perm_cap_ = 0;
+  void* pm_ = nullptr;     size_t pm_cap_ = 0;   // [Npad] non-constant dimensions per point
 
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;

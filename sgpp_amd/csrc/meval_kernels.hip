@@ -99,17 +99,18 @@ std::vector<int32_t> grid_order(const std::vector<double>& li, const std::vector
   return order;
 }
 
-// Per 64-point tile and dimension the union of the points' supports
-// [(I - 1) / L, (I + 1) / L] (whole line for L = 0), from point-major (L, I)
-// pairs given as doubles, stored in T rounded outwards.
+// Per 64-point tile and dimension the union of the supports of its points
+// below n, [(I - 1) / L, (I + 1) / L] (whole line for L = 0), from
+// point-major (L, I) pairs given as doubles, stored in T rounded outwards; a
+// tile with no such point gets an empty box.
 template <typename T>
-std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d) {
+std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t n, size_t d) {
   const size_t ntiles = npad / kTG;
   std::vector<T> tb(ntiles * d * 2);
   for (size_t t = 0; t < ntiles; ++t)
     for (size_t k = 0; k < d; ++k) {
       double lo = 2.0, hi = -1.0;
-      for (size_t p = 0; p < static_cast<size_t>(kTG); ++p) {
+      for (size_t p = 0; p < static_cast<size_t>(kTG) && t * kTG + p < n; ++p) {
         const size_t e = ((t * kTG + p) * d + k) * 2;
         const double L = li[e], I = li[e + 1];
         if (L > 0.0) {
@@ -129,8 +130,9 @@ std::vector<T> tile_boxes(const std::vector<double>& li, size_t npad, size_t d) 
   return tb;
 }
 
-template std::vector<float> tile_boxes<float>(const std::vector<double>&, size_t, size_t);
-template std::vector<double> tile_boxes<double>(const std::vector<double>&, size_t, size_t);
+template std::vector<float> tile_boxes<float>(const std::vector<double>&, size_t, size_t, size_t);
+template std::vector<double> tile_boxes<double>(const std::vector<double>&, size_t, size_t,
+                                                size_t);
 
 // =================================================================== runtime
 
@@ -224,7 +226,7 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, dperm_, A_, Y_,
+  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, pm_, dperm_, A_, Y_,
                   part_, hin_, hout_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -275,12 +277,25 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
     sc[s] = sc0[j];
   }
   const size_t Npad = ((N + kGridPad - 1) / kGridPad) * kGridPad;
-  // padded points: (L, I) = (0, 0) -> factor 1, weight 0 through A / ignored rows
+  // padded points: (L, I) = (1, 4) in every dimension, a factor that is
+  // exactly 0 on [0, 1] (u = x - 4), so the culling tests drop them; weight 0
+  // through A, rows ignored by the reductions
   li.resize(Npad * D_ * 2, 0.0);
+  for (size_t s = N; s < Npad; ++s)
+    for (size_t d = 0; d < D_; ++d) {
+      li[(s * D_ + d) * 2] = 1.0;
+      li[(s * D_ + d) * 2 + 1] = 4.0;
+    }
   sc.resize(Npad, 1.0);
+  // non-constant dimensions per stored point (0 for padding)
+  std::vector<uint32_t> pm(Npad, 0u);
+  for (size_t s = 0; s < N; ++s) pm[s] = pmask[order[s]];
   check_hip(hipStreamSynchronize(stream_), "hipStreamSynchronize");
   N_ = N;
   Npad_ = Npad;
+  ensure(&pm_, &pm_cap_, Npad_ * sizeof(uint32_t));
+  check_hip(hipMemcpy(pm_, pm.data(), Npad_ * sizeof(uint32_t), hipMemcpyHostToDevice),
+            "upload dimension masks");
   ensure(reinterpret_cast<void**>(&scale_), &gridcap_scale_, Npad_ * sizeof(double));
   check_hip(hipMemcpy(scale_, sc.data(), Npad_ * sizeof(double), hipMemcpyHostToDevice),
             "upload scale");
@@ -292,7 +307,7 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
     check_hip(hipMemcpy(LI64_, li.data(), li.size() * sizeof(double), hipMemcpyHostToDevice),
               "upload LI64");
     // FP32 boxes of the FP64 supports, rounded outwards
-    const std::vector<float> tb = tile_boxes<float>(li, Npad_, D_);
+    const std::vector<float> tb = tile_boxes<float>(li, Npad_, N_, D_);
     ensure(&TB64_, &tbcap64_, tb.size() * sizeof(float));
     check_hip(hipMemcpy(TB64_, tb.data(), tb.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload tile boxes");
@@ -304,7 +319,7 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
               "upload LI32");
     // boxes of the FP32 (L, I) the FP32 kernels evaluate
     const std::vector<double> l32d(l32.begin(), l32.end());
-    const std::vector<float> tb = tile_boxes<float>(l32d, Npad_, D_);
+    const std::vector<float> tb = tile_boxes<float>(l32d, Npad_, N_, D_);
     ensure(&TB32_, &tbcap32_, tb.size() * sizeof(float));
     check_hip(hipMemcpy(TB32_, tb.data(), tb.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload tile boxes");
@@ -372,8 +387,8 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
     const bool skip = basis_ == Basis::ModLinear;
     MultFn<T> f = count_ ? count_mult_launcher<T>(static_cast<int>(D_), skip)
                          : DimTable<T>::mult(static_cast<int>(D_), skip);
-    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const T*>(A),
-      P, chunk, jend, G, static_cast<int64_t>(Mpad_), count_, s);
+    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const uint32_t*>(pm_),
+      static_cast<const T*>(A), P, chunk, jend, G, static_cast<int64_t>(Mpad_), count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
     clear_launch_error();
@@ -433,7 +448,8 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
     MultTFn<T> f = count_ ? count_multT_launcher<T>(static_cast<int>(D_), skip)
                           : DimTable<T>::multT(static_cast<int>(D_), skip);
     f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
-      P, static_cast<int64_t>(Npad_), jend, seg, mend, S, count_, s);
+      static_cast<const uint32_t*>(pm_), P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
+      count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(S));
     clear_launch_error();

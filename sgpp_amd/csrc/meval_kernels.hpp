@@ -85,8 +85,8 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * 
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
 void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
-                 const float* __restrict__ TB, const T* __restrict__ A,
-                 double* __restrict__ part, int chunk, int jend_all,
+                 const float* __restrict__ TB, const uint32_t* __restrict__ PM,
+                 const T* __restrict__ A, double* __restrict__ part, int chunk, int jend_all,
                  unsigned long long* __restrict__ cnt) {
   // a wave's 64*K points are contiguous (a small box in Morton order): point
   // k of a lane is m0 + 64*k
@@ -110,6 +110,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
     if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
     const T* tile = LI + t * D * kTG * 2;
     unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
+    const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;  // this lane's point's dimensions
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
@@ -119,7 +120,8 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
       T a[J];
       take_hits<J, true, T>(bal, A, j0, jj, used, a);
       T p[J][K];
-      const int ev = eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
+      const int ev =
+          eval_group<D, J, K, T, kSkip>(tile, jj, x, p, kSkip ? group_dims<J>(pm, jj) : ~0u);
       if (kCount) {
         n_steps += static_cast<unsigned long long>(ev) * (J * K * 64);
         n_pairs += J * K * 64;
@@ -182,8 +184,8 @@ template <int D, int K, int J, typename T, bool kSkip, bool kCount>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
 void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
                   const T* __restrict__ LI, const float* __restrict__ TB,
-                  double* __restrict__ part, int64_t ldp, int jend_all, int64_t seg,
-                  int64_t mend, unsigned long long* __restrict__ cnt) {
+                  const uint32_t* __restrict__ PM, double* __restrict__ part, int64_t ldp,
+                  int jend_all, int64_t seg, int64_t mend, unsigned long long* __restrict__ cnt) {
   __shared__ double accs[kBlock / 64][kChunkT];
   const int wave = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -215,6 +217,7 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
       if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
       const T* tile = LI + t * D * kTG * 2;
       unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
+      const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;  // this lane's point's dimensions
       // a grid point is taken at most once per (group, tile): its sum goes
       // to lane jj of tsum, and the tile is added to the wave's row once
       T tsum = T(0);
@@ -224,7 +227,8 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
         T unused_w[J];
         take_hits<J, false, T>(bal, static_cast<const T*>(nullptr), j0, jj, used, unused_w);
         T p[J][K];
-        const int ev = eval_group<D, J, K, T, kSkip>(tile, jj, x, p);
+        const int ev =
+            eval_group<D, J, K, T, kSkip>(tile, jj, x, p, kSkip ? group_dims<J>(pm, jj) : ~0u);
         if (kCount) {
           n_steps += static_cast<unsigned long long>(ev) * (J * K * 64);
           n_pairs += J * K * 64;
@@ -345,19 +349,19 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 // kSkip: test for constant dimensions (ModLinear); Linear grids have none.
 // kCount: the work-counting instantiation (cnt: 4 counters).
 template <typename T, int D, bool kSkip, bool kCount = false>
-void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const T* A,
-                   double* part, int chunk, int jend, int G, int64_t Mpad,
+void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const uint32_t* PM,
+                   const T* A, double* part, int chunk, int jend, int G, int64_t Mpad,
                    unsigned long long* cnt, hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
   hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock), 0,
-                     s, X, ldx, LI, TB, A, part, chunk, jend, cnt);
+                     s, X, ldx, LI, TB, PM, A, part, chunk, jend, cnt);
 }
 
 template <typename T, int D, bool kSkip, bool kCount = false>
 void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const float* TB,
-                    double* part, int64_t ldp, int jend, int64_t seg,
+                    const uint32_t* PM, double* part, int64_t ldp, int jend, int64_t seg,
                     int64_t mend, int S, unsigned long long* cnt, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
   // data segments vary fastest: consecutive blocks share one grid chunk
@@ -365,15 +369,16 @@ void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const floa
   dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>((jend + kChunkT - 1) / kChunkT));
   clear_launch_error();
   hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock),
-                     0, s, X, ldx, Y, LI, TB, part, ldp, jend, seg, mend, cnt);
+                     0, s, X, ldx, Y, LI, TB, PM, part, ldp, jend, seg, mend, cnt);
 }
 
 template <typename T>
-using MultFn = void (*)(const T*, int64_t, const T*, const float*, const T*, double*, int, int,
-                        int, int64_t, unsigned long long*, hipStream_t);
+using MultFn = void (*)(const T*, int64_t, const T*, const float*, const uint32_t*, const T*,
+                        double*, int, int, int, int64_t, unsigned long long*, hipStream_t);
 template <typename T>
-using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, double*, int64_t,
-                         int, int64_t, int64_t, int, unsigned long long*, hipStream_t);
+using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, const uint32_t*,
+                         double*, int64_t, int, int64_t, int64_t, int, unsigned long long*,
+                         hipStream_t);
 
 template <typename T, bool kCount, int... Ds>
 struct Table {
@@ -416,11 +421,12 @@ int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
 // dimension) and pairs (J slots x K points x 64 lanes per group) to cnt[0],
 // cnt[1] (mult) or cnt[2], cnt[3] (multTranspose).
 template <typename T>
-using CountMultFn = void (*)(const T*, int64_t, const T*, const float*, const T*, double*, int, int,
-                             int, int64_t, unsigned long long*, hipStream_t);
+using CountMultFn = void (*)(const T*, int64_t, const T*, const float*, const uint32_t*, const T*,
+                             double*, int, int, int, int64_t, unsigned long long*, hipStream_t);
 template <typename T>
-using CountMultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, double*, int64_t,
-                              int, int64_t, int64_t, int, unsigned long long*, hipStream_t);
+using CountMultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*,
+                              const uint32_t*, double*, int64_t, int, int64_t, int64_t, int,
+                              unsigned long long*, hipStream_t);
 template <typename T>
 CountMultFn<T> count_mult_launcher(int D, bool skip);
 template <typename T>
