@@ -399,6 +399,7 @@ def solve_bench(args):
     import torch
 
     from sgpp_amd import capi, workloads
+    copy_reference(args.copy_reference_mib)
     c = workloads.config_regression()
     lv, ix = c["grid"].level_index()
     X, y, lam = c["X"], c["y"], c["lam"]
@@ -445,10 +446,43 @@ def solve_bench(args):
         "mixed_refinement": runs["mixed_refinement"],
         "cpu_reference": ({"iterations": int(ref["iterations"]), "seconds": float(ref["seconds"]),
                            "threads": int(ref["threads"]),
-                           "solver": "reference ConjugateGradients on OperationMultipleEvalSubspaceCombined"}
+                           "solver": "reference ConjugateGradients on OperationMultipleEvalSubspaceCombined",
+                           "where": "build container, when the fixture was written"}
                           if ref is not None else None),
     }
+    if not args.no_cpu:
+        line["cpu_reference_this_host"] = cpu_solve_slice(c, lv, ix,
+                                                          int(ref["iterations"]) if ref is not None
+                                                          else None)
     print(json.dumps(line))
+
+
+def cpu_solve_slice(c, lv, ix, iterations):
+    """The reference's FP64 CPU solve (ConjugateGradients on its SubspaceLinear
+    operator, compiled from its sources into oracle/_ref) timed on this host:
+    one CG iteration and four (each including b = B^T y and the start
+    residual), whose difference gives the time per iteration; the full solve is
+    projected from the fixture's iteration count."""
+    import oracle
+    if oracle.ref() is None:
+        return None
+    g = oracle.RefGrid.from_arrays(0, lv, ix)
+    t = time.perf_counter()
+    op = oracle.RefOp(g, oracle.KIND_SUBSPACE, c["X"])
+    setup = time.perf_counter() - t
+    times = {}
+    for k in (1, 4):
+        t = time.perf_counter()
+        op.cg_solve(c["y"], c["lam"], k, 1e-10)
+        times[k] = time.perf_counter() - t
+    per_it = (times[4] - times[1]) / 3.0
+    out = {"threads": oracle.num_threads(), "cpu": cpu_model(), "operator_setup_s": setup,
+           "solve_1_iteration_s": times[1], "solve_4_iterations_s": times[4],
+           "seconds_per_iteration": per_it}
+    if iterations:
+        out["projected_solve_s"] = times[1] + (iterations - 1) * per_it
+        out["projected_iterations"] = iterations
+    return out
 
 
 def large_bench(args):
@@ -464,6 +498,7 @@ def large_bench(args):
     import oracle
     from sgpp_amd import capi, workloads
     t_start = time.perf_counter()
+    copy_reference(args.copy_reference_mib)
 
     def note(msg):
         print("[large] %s (%.0f s)" % (msg, time.perf_counter() - t_start), file=sys.stderr,
@@ -542,6 +577,20 @@ def large_bench(args):
     print(json.dumps(line))
 
 
+def copy_reference(mib):
+    """mib MiB copied device to device three times: a kernel of known traffic
+    (__amd_rocclr_copyBuffer) for scaling the hardware byte counters"""
+    if mib <= 0:
+        return
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -575,13 +624,7 @@ def main():
     op = capi.MultiEval(capi.BASIS_MODLINEAR, X, precision=capi.PREC_FP32, keep_fp64=True)
     op.prepare(lv, ix)
     setup_s = time.perf_counter() - t_setup
-    if args.copy_reference_mib > 0:
-        src = torch.ones(args.copy_reference_mib << 18, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_reference(args.copy_reference_mib)
 
     res = {}
     outs = {}
