@@ -99,7 +99,21 @@ typedef struct sgpp_hip_params {
   int mult_grid_chunks; /* 0 = automatic */
   int multT_data_segs;  /* 0 = automatic */
   int target_blocks;    /* 0 = automatic */
+  int algorithm;        /* SGPP_HIP_ALGO_* (0 = streaming) */
 } sgpp_hip_params;
+
+/* STREAMING: every (data point, grid point) pair, culled by supports
+ *   (OperationMultiEvalStreaming / OperationMultiEvalModMaskStreaming);
+ * SUBSPACE: per data point and subspace the one basis function whose support
+ *   holds the point, looked up in a dense per-subspace index table
+ *   (OperationMultipleEvalSubspaceCombined, datadriven/src/sgpp/datadriven/
+ *   operation/hash/OperationMultipleEvalSubspace/combined/
+ *   OperationMultipleEvalSubspaceCombined.hpp:92-204). FP64 only
+ *   (precision SGPP_HIP_PREC_FP64, keep_fp64 0), dimension <= 20, Linear and
+ *   ModLinear; multiTranspose sums with FP64 atomics, so its last bits vary
+ *   between runs. Grids whose index tables would exceed
+ *   max(2^26, 64 N) entries are refused. */
+enum { SGPP_HIP_ALGO_STREAMING = 0, SGPP_HIP_ALGO_SUBSPACE = 1 };
 
 /* constructor: copies, pads and transposes the dataset into HBM
  * (OperationMultiEvalStreaming.cpp:13-25) */

@@ -19,6 +19,12 @@
 //   "INTERNAL_PRECISION": "double" (default) or "float" (FP32 products with
 //                         FP64 accumulation),
 //   "DEVICE":             HIP device ordinal (default 0).
+// The factory's SUBSPACELINEAR type with the HIP sub-type selects the
+// subspace-indexed kernels (the SubspaceLinear semantics, FP64 only):
+//
+//   if (configuration.getType() == datadriven::OperationMultipleEvalType::SUBSPACELINEAR &&
+//       configuration.getSubType() == datadriven::kOperationMultipleEvalSubTypeHIP)
+//     return datadriven::createOperationMultiEvalHIP(grid, dataset, configuration);
 #pragma once
 
 #include <sgpp/base/datatypes/DataMatrix.hpp>
@@ -53,10 +59,10 @@ class OperationMultiEvalHIP : public base::OperationMultipleEval {
   // Copies the dataset into HBM (the streaming constructor's copy, pad and
   // transpose, OperationMultiEvalStreaming.cpp:13-25) and reads the grid.
   OperationMultiEvalHIP(base::Grid& grid, base::DataMatrix& dataset, int basis, int precision,
-                        int device)
-      : OperationMultipleEval(grid, dataset) {
+                        int device, int algorithm = SGPP_HIP_ALGO_STREAMING)
+      : OperationMultipleEval(grid, dataset), algorithm_(algorithm) {
     sgpp_hip_grid_desc desc{basis, grid.getDimension()};
-    sgpp_hip_params p{precision, /*keep_fp64=*/0, device, 0, 0, 0};
+    sgpp_hip_params p{precision, /*keep_fp64=*/0, device, 0, 0, 0, algorithm};
     check(sgpp_hip_meval_create(&desc, dataset.getPointer(), dataset.getNrows(),
                                 dataset.getNcols(), &p, &h_));
     try {
@@ -134,7 +140,9 @@ class OperationMultiEvalHIP : public base::OperationMultipleEval {
 
   double getDuration() override { return sgpp_hip_meval_duration(h_); }
 
-  std::string getImplementationName() override { return "HIP_STREAMING"; }
+  std::string getImplementationName() override {
+    return algorithm_ == SGPP_HIP_ALGO_SUBSPACE ? "HIP_SUBSPACE" : "HIP_STREAMING";
+  }
 
  private:
   // status codes back to SG++'s exception types (sgpp_hip.h)
@@ -154,6 +162,7 @@ class OperationMultiEvalHIP : public base::OperationMultipleEval {
   }
 
   sgpp_hip_meval* h_ = nullptr;
+  int algorithm_;
 };
 
 // The factory branch: Linear and ModLinear grids; any other grid type is a
@@ -182,6 +191,13 @@ inline base::OperationMultipleEval* createOperationMultiEvalHIP(
             "OperationMultiEvalHIP: INTERNAL_PRECISION must be \"double\" or \"float\"");
     }
     if (params->contains("DEVICE")) device = static_cast<int>((*params)["DEVICE"].getInt());
+  }
+  if (configuration.getType() == OperationMultipleEvalType::SUBSPACELINEAR) {
+    if (precision != SGPP_HIP_PREC_FP64)
+      throw base::factory_exception(
+          "OperationMultiEvalHIP: the SUBSPACELINEAR kernels compute in double precision");
+    return new OperationMultiEvalHIP(grid, dataset, basis, precision, device,
+                                     SGPP_HIP_ALGO_SUBSPACE);
   }
   return new OperationMultiEvalHIP(grid, dataset, basis, precision, device);
 }

@@ -16,7 +16,8 @@ LIB = os.path.join(HERE, "libsgpp_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["meval_kernels.hip", "meval_count.hip", "solver_kernels.hip", "server.hip"]
+HIP_SOURCES = ["meval_kernels.hip", "meval_count.hip", "subspace_kernels.hip", "solver_kernels.hip",
+               "server.hip"]
 CPP_SOURCES = ["grid.cpp", "operation_hip.cpp", "learner.cpp", "capi.cpp", "server_capi.cpp"]
 
 INCLUDES = ["-I" + CSRC, "-I" + os.path.join(CSRC, "include"),

@@ -22,6 +22,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "FACTORY", 3: "OPERATION", 4:
                 10: "TIMEOUT", 11: "SOLVER", 99: "INTERNAL"}
 BASIS_LINEAR, BASIS_MODLINEAR = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
+ALGO_STREAMING, ALGO_SUBSPACE = 0, 1
 SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
 SLE_CG, SLE_BICGSTAB = 0, 1
 
@@ -62,7 +63,7 @@ class GridDesc(ctypes.Structure):
 class Params(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("keep_fp64", ctypes.c_int), ("device", ctypes.c_int),
                 ("mult_grid_chunks", ctypes.c_int), ("multT_data_segs", ctypes.c_int),
-                ("target_blocks", ctypes.c_int)]
+                ("target_blocks", ctypes.c_int), ("algorithm", ctypes.c_int)]
 
 
 class SolveParams(ctypes.Structure):
@@ -330,17 +331,22 @@ class Grid:
 
 
 class MultiEval:
-    """OperationMultipleEval on the GPU (mult = B alpha, multTranspose = B^T y)."""
+    """OperationMultipleEval on the GPU (mult = B alpha, multTranspose = B^T y).
+
+    algorithm=ALGO_STREAMING: the culled streaming kernels (FP64 or FP32 with
+    FP64 accumulation); ALGO_SUBSPACE: the subspace-indexed kernels
+    (SubspaceLinear semantics, FP64, d <= 20)."""
 
     def __init__(self, basis, data, precision=PREC_FP64, keep_fp64=False, device=0,
-                 mult_grid_chunks=0, multT_data_segs=0, target_blocks=0, grid=None):
+                 mult_grid_chunks=0, multT_data_segs=0, target_blocks=0, grid=None,
+                 algorithm=ALGO_STREAMING):
         data = _f64(data)
         if data.ndim != 2:
             raise ValueError("data must be M x d")
         self.M, self.d = data.shape
         self.precision = precision
         prm = Params(precision, int(keep_fp64), device, mult_grid_chunks, multT_data_segs,
-                     target_blocks)
+                     target_blocks, algorithm)
         h = _p()
         if grid is not None:
             self.grid = grid  # keep alive

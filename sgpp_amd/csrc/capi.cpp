@@ -103,6 +103,11 @@ sgpp::datadriven::HIPOperationConfiguration hip_config_of(const sgpp_hip_params*
     hc.multGridChunks = params->mult_grid_chunks;
     hc.multTDataSegs = params->multT_data_segs;
     hc.targetBlocks = params->target_blocks;
+    if (params->algorithm != SGPP_HIP_ALGO_STREAMING && params->algorithm != SGPP_HIP_ALGO_SUBSPACE)
+      throw sgpp::base::factory_exception("unknown algorithm");
+    hc.algorithm = params->algorithm == SGPP_HIP_ALGO_SUBSPACE
+                       ? sgpp::datadriven::HIPAlgorithm::SUBSPACE
+                       : sgpp::datadriven::HIPAlgorithm::STREAMING;
   }
   return hc;
 }
@@ -287,12 +292,17 @@ sgpp_hip_status sgpp_hip_meval_create(const sgpp_hip_grid_desc* grid, const doub
     const sgpp_hip::Precision pr = prec_of(precision);
     unsigned mask = 1u << static_cast<int>(pr);
     if (params && params->keep_fp64) mask |= 1u;
+    const int algorithm = params ? params->algorithm : SGPP_HIP_ALGO_STREAMING;
+    if (algorithm != SGPP_HIP_ALGO_STREAMING && algorithm != SGPP_HIP_ALGO_SUBSPACE)
+      throw sgpp::base::factory_exception("unknown algorithm");
     auto h = std::unique_ptr<sgpp_hip_meval>(new sgpp_hip_meval());
     h->own.reset(new sgpp_hip::DeviceMultiEval(
         data, M, d,
         grid->basis == SGPP_HIP_BASIS_LINEAR ? sgpp_hip::Basis::Linear
                                              : sgpp_hip::Basis::ModLinear,
-        mask, params ? params->device : 0, launch_of(params)));
+        mask, params ? params->device : 0, launch_of(params),
+        algorithm == SGPP_HIP_ALGO_SUBSPACE ? sgpp_hip::Algorithm::Subspace
+                                            : sgpp_hip::Algorithm::Streaming));
     h->dev = h->own.get();
     h->prec = pr;
     *out = h.release();
@@ -308,9 +318,13 @@ sgpp_hip_status sgpp_hip_meval_create_for_grid(sgpp_hip_grid* grid, const double
     auto h = std::unique_ptr<sgpp_hip_meval>(new sgpp_hip_meval());
     const size_t d = grid->grid->getDimension();
     h->data = DataMatrix(data, M, d);
-    const sgpp::datadriven::HIPOperationConfiguration hc = hip_config_of(params);
+    sgpp::datadriven::HIPOperationConfiguration hc = hip_config_of(params);
+    // the subspace algorithm through the factory's SUBSPACELINEAR branch
+    const bool subspace = hc.algorithm == sgpp::datadriven::HIPAlgorithm::SUBSPACE;
+    hc.algorithm = sgpp::datadriven::HIPAlgorithm::STREAMING;
     sgpp::datadriven::OperationMultipleEvalConfiguration cfg(
-        sgpp::datadriven::OperationMultipleEvalType::STREAMING,
+        subspace ? sgpp::datadriven::OperationMultipleEvalType::SUBSPACELINEAR
+                 : sgpp::datadriven::OperationMultipleEvalType::STREAMING,
         sgpp::datadriven::OperationMultipleEvalSubType::HIP, hc, "HIP");
     h->op.reset(sgpp::op_factory::createOperationMultipleEval(*grid->grid, h->data, cfg));
     auto* hop = dynamic_cast<sgpp::datadriven::OperationMultiEvalHIP*>(h->op.get());

@@ -52,6 +52,15 @@ enum class OperationMultipleEvalMPIType { NONE, MASTERSLAVE, HPX };
 //                  partials and all reductions accumulated in FP64
 enum class HIPPrecision { FP64 = 0, FP32_FP64ACC = 1 };
 
+// Evaluation scheme of the HIP operator.
+//   STREAMING: every (data point, grid point) pair, culled by supports
+//              (the StreamingDefault / ModMaskStreaming semantics)
+//   SUBSPACE:  per data point and subspace the one basis function whose
+//              support holds the point, from a dense index table (the
+//              SubspaceLinear semantics; FP64, dimension <= 20). The factory
+//              selects it for OperationMultipleEvalType::SUBSPACELINEAR.
+enum class HIPAlgorithm { STREAMING = 0, SUBSPACE = 1 };
+
 class HIPOperationConfiguration : public base::OperationConfiguration {
  public:
   HIPPrecision precision = HIPPrecision::FP64;
@@ -62,6 +71,7 @@ class HIPOperationConfiguration : public base::OperationConfiguration {
   int multGridChunks = 0;   // 0 = automatic
   int multTDataSegs = 0;    // 0 = automatic
   int targetBlocks = 0;     // 0 = automatic
+  HIPAlgorithm algorithm = HIPAlgorithm::STREAMING;
   base::OperationConfiguration* clone() const override {
     return new HIPOperationConfiguration(*this);
   }

@@ -20,6 +20,10 @@ namespace sgpp_hip {
 
 enum class Basis : int { Linear = 0, ModLinear = 1 };
 enum class Precision : int { FP64 = 0, FP32 = 1 };
+// Streaming: every (data point, grid point) pair, culled by supports.
+// Subspace: per data point and subspace (level vector), the one basis function
+//   whose support holds the point, looked up in a dense index table (FP64).
+enum class Algorithm : int { Streaming = 0, Subspace = 1 };
 
 // Error thrown for any failing HIP call or invalid launch shape.
 struct device_error : public std::runtime_error {
@@ -80,8 +84,11 @@ class DeviceMultiEval {
  public:
   // data_rowmajor: M x d, FP64 row-major (the caller's DataMatrix layout).
   // precisions: bitmask of (1 << Precision) that the instance must serve.
+  // The subspace algorithm computes in FP64 only (precisions must be FP64)
+  // and supports dim <= 20.
   DeviceMultiEval(const double* data_rowmajor, size_t M, size_t dim, Basis basis,
-                  unsigned precisions, int device, const LaunchConfig& cfg);
+                  unsigned precisions, int device, const LaunchConfig& cfg,
+                  Algorithm algorithm = Algorithm::Streaming);
   ~DeviceMultiEval();
   DeviceMultiEval(const DeviceMultiEval&) = delete;
   DeviceMultiEval& operator=(const DeviceMultiEval&) = delete;
@@ -111,6 +118,10 @@ class DeviceMultiEval {
   size_t num_grid() const { return N_; }
   size_t dim() const { return D_; }
   Basis basis() const { return basis_; }
+  Algorithm algorithm() const { return algorithm_; }
+  // subspace algorithm: number of subspaces and of index table entries
+  int num_subspaces() const { return nsub_; }
+  int64_t num_table_entries() const { return ntable_; }
   bool serves(Precision p) const { return (precisions_ >> static_cast<int>(p)) & 1u; }
 
   // Kernel-only milliseconds of the last mult / multTranspose (HIP events).
@@ -153,6 +164,10 @@ class DeviceMultiEval {
   template <typename T>
   void run_multT(const T* X, const T* LI, const float* TB, const double* d_y, double* d_result,
                  hipEvent_t* tev);
+  // the subspace algorithm (subspace_kernels.hip)
+  void set_grid_subspace(const uint32_t* level, const uint32_t* index, size_t N);
+  void run_subspace_mult(const double* d_alpha, double* d_result, hipEvent_t* tev);
+  void run_subspace_multT(const double* d_y, double* d_result, hipEvent_t* tev);
 
   // Storage orders. Data: Morton order, dperm_[stored row] = caller's row.
   // Grid: by each point's set of non-constant dimensions, then Morton order
@@ -164,6 +179,7 @@ class DeviceMultiEval {
   size_t M_ = 0, Mpad_ = 0, D_ = 0, N_ = 0, Npad_ = 0;
   Basis basis_;
   unsigned precisions_;
+  Algorithm algorithm_ = Algorithm::Streaming;
   int device_ = 0;
   LaunchConfig cfg_;
   int num_cus_ = 256;
@@ -182,6 +198,21 @@ class DeviceMultiEval {
   void* TB32_ = nullptr;
   double* scale_ = nullptr;
   size_t gridcap_li64_ = 0, gridcap_li32_ = 0, gridcap_scale_ = 0, tbcap64_ = 0, tbcap32_ = 0;
+
+  // subspace algorithm: subspace records (sorted by level vector), the index
+  // tables (grid point per cell, -1 where the grid has none), each grid
+  // point's table entry, and a table-shaped FP64 buffer (weights for mult,
+  // sums for multTranspose); sub_recompute_ = sum over subspaces of the
+  // dimensions not shared with the previous one
+  int nsub_ = 0;
+  int64_t ntable_ = 0;
+  uint64_t sub_recompute_ = 0;
+  void* sub_desc_ = nullptr;  size_t sub_desc_cap_ = 0;
+  void* sub_ti_ = nullptr;    size_t sub_ti_cap_ = 0;
+  void* sub_te_ = nullptr;    size_t sub_te_cap_ = 0;
+  void* sub_tw_ = nullptr;    size_t sub_tw_cap_ = 0;
+  void* sub_chunk_ = nullptr; size_t sub_chunk_cap_ = 0;  // multTranspose chunks
+  int nchunk_ = 0;
 
   // scratch
   void* A_ = nullptr;     size_t A_cap_ = 0;      // alpha*scale in compute type

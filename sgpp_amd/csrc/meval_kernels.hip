@@ -179,11 +179,22 @@ void encode_grid(Basis basis, size_t dim, size_t n, const uint32_t* level, const
 }
 
 DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis basis,
-                                 unsigned precisions, int device, const LaunchConfig& cfg)
-    : M_(M), D_(dim), basis_(basis), precisions_(precisions), device_(device), cfg_(cfg) {
+                                 unsigned precisions, int device, const LaunchConfig& cfg,
+                                 Algorithm algorithm)
+    : M_(M),
+      D_(dim),
+      basis_(basis),
+      precisions_(precisions),
+      algorithm_(algorithm),
+      device_(device),
+      cfg_(cfg) {
   if (M == 0 || dim == 0) throw input_error("empty dataset");
   if (precisions == 0 || (precisions & ~3u)) throw input_error("invalid precision mask");
   if (dim > 64) throw input_error("dimension above 64 is not supported");
+  if (algorithm == Algorithm::Subspace && precisions != 1u)
+    throw input_error("the subspace algorithm computes in FP64 only");
+  if (algorithm == Algorithm::Subspace && dim > 20)
+    throw input_error("the subspace algorithm supports dimensions up to 20");
   check_hip(hipSetDevice(device_), "hipSetDevice");
   hipDeviceProp_t prop;
   check_hip(hipGetDeviceProperties(&prop, device_), "hipGetDeviceProperties");
@@ -226,8 +237,9 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_, X32_, LI64_, LI32_, TB64_, TB32_, scale_, perm_, pm_, dperm_, A_, Y_,
-                  part_, hin_, hout_};
+  void* bufs[] = {X64_,     X32_,    LI64_,   LI32_,   TB64_, TB32_, scale_, perm_,
+                  pm_,      dperm_,  A_,      Y_,      part_, hin_,  hout_,  sub_desc_,
+                  sub_ti_,  sub_te_, sub_tw_, sub_chunk_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto* pool : {&tm_mult_, &tm_multT_})
@@ -255,6 +267,10 @@ void DeviceMultiEval::ensure(void** buf, size_t* cap, size_t bytes) {
 void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, size_t N) {
   if (N == 0) throw input_error("empty grid");
   if (N > (size_t(1) << 30)) throw input_error("grid too large");
+  if (algorithm_ == Algorithm::Subspace) {
+    set_grid_subspace(level, index, N);
+    return;
+  }
   // non-constant dimensions of each point (ModLinear level-1 factors are 1)
   const bool mod = basis_ == Basis::ModLinear;
   std::vector<uint32_t> pmask(N);
@@ -475,7 +491,9 @@ void DeviceMultiEval::mult_device(const double* d_alpha, double* d_result, Preci
   EventPair* tp = timing_on_ ? next_timer(tm_mult_, used_mult_) : nullptr;
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
-  if (p == Precision::FP64)
+  if (algorithm_ == Algorithm::Subspace)
+    run_subspace_mult(d_alpha, d_result, tp ? tev : nullptr);
+  else if (p == Precision::FP64)
     run_mult<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
                      static_cast<const float*>(TB64_), d_alpha, d_result, tp ? tev : nullptr);
   else
@@ -514,6 +532,8 @@ void* device_scratch(int device) {
 void DeviceMultiEval::mult_device_noalloc(const double* d_alpha, double* d_result, Precision p) {
   if (N_ == 0) throw input_error("grid not set (call prepare)");
   if (!serves(p)) throw input_error("precision not enabled for this instance");
+  if (algorithm_ != Algorithm::Streaming)
+    throw input_error("the no-allocation entry point serves the streaming algorithm only");
   const MultScratch sc{device_scratch(device_), kNoAllocScratchBytes};
   check_hip(hipSetDevice(device_), "hipSetDevice");
   record_start();
@@ -533,7 +553,9 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
   EventPair* tp = timing_on_ ? next_timer(tm_multT_, used_multT_) : nullptr;
   hipEvent_t tev[2] = {tp ? tp->a : nullptr, tp ? tp->b : nullptr};
   record_start();
-  if (p == Precision::FP64)
+  if (algorithm_ == Algorithm::Subspace)
+    run_subspace_multT(d_y, d_result, tp ? tev : nullptr);
+  else if (p == Precision::FP64)
     run_multT<double>(static_cast<const double*>(X64_), static_cast<const double*>(LI64_),
                       static_cast<const float*>(TB64_), d_y, d_result, tp ? tev : nullptr);
   else
@@ -545,6 +567,13 @@ void DeviceMultiEval::mult_transpose_device(const double* d_y, double* d_result,
 void DeviceMultiEval::count_work(Precision p, uint64_t out[4]) {
   if (N_ == 0) throw input_error("grid not set (call prepare)");
   if (!serves(p)) throw input_error("precision not enabled for this instance");
+  if (algorithm_ == Algorithm::Subspace) {
+    // every data row (padding included) visits every subspace and recomputes
+    // the dimensions it does not share with the previous one
+    out[1] = out[3] = static_cast<uint64_t>(Mpad_) * static_cast<uint64_t>(nsub_);
+    out[0] = out[2] = static_cast<uint64_t>(Mpad_) * sub_recompute_;
+    return;
+  }
   check_hip(hipSetDevice(device_), "hipSetDevice");
   const size_t n = std::max(N_, M_) * sizeof(double);
   ensure(&hin_, &hin_cap_, n);

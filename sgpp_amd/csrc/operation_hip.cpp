@@ -34,8 +34,10 @@ OperationMultiEvalHIP::OperationMultiEvalHIP(base::Grid& grid, base::DataMatrix&
                                     ? sgpp_hip::Basis::Linear
                                     : sgpp_hip::Basis::ModLinear;
   numData_ = dataset.getNrows();
-  dev_.reset(new sgpp_hip::DeviceMultiEval(dataset.getPointer(), numData_, dataset.getNcols(),
-                                           basis, mask, config.device, lc));
+  dev_.reset(new sgpp_hip::DeviceMultiEval(
+      dataset.getPointer(), numData_, dataset.getNcols(), basis, mask, config.device, lc,
+      config.algorithm == HIPAlgorithm::SUBSPACE ? sgpp_hip::Algorithm::Subspace
+                                                 : sgpp_hip::Algorithm::Streaming));
   this->prepare();
 }
 
@@ -96,6 +98,7 @@ double OperationMultiEvalHIP::getKernelMilliseconds() {
 }
 
 std::string OperationMultiEvalHIP::getImplementationName() {
+  if (config_.algorithm == HIPAlgorithm::SUBSPACE) return "HIP_SUBSPACE_FP64";
   return config_.precision == HIPPrecision::FP64 ? "HIP_STREAMING_FP64"
                                                  : "HIP_STREAMING_FP32_FP64ACC";
 }
@@ -113,8 +116,12 @@ base::OperationMultipleEval* createOperationMultipleEval(
   if (configuration.getMPIType() != OperationMultipleEvalMPIType::NONE)
     throw base::factory_exception(
         "Error creating function: the library wasn't compiled with MPI/HPX support");
+  // (STREAMING or DEFAULT, HIP): the streaming kernels
+  // (SUBSPACELINEAR, HIP): the subspace kernels, where the reference's factory
+  //   creates OperationMultipleEvalSubspaceCombined (DatadrivenOpFactory.cpp:453-468)
+  const bool subspace = configuration.getType() == OperationMultipleEvalType::SUBSPACELINEAR;
   if (configuration.getSubType() == OperationMultipleEvalSubType::HIP &&
-      (configuration.getType() == OperationMultipleEvalType::STREAMING ||
+      (subspace || configuration.getType() == OperationMultipleEvalType::STREAMING ||
        configuration.getType() == OperationMultipleEvalType::DEFAULT)) {
     if (grid.getType() != base::GridType::Linear && grid.getType() != base::GridType::ModLinear)
       throw base::factory_exception("OperationMultiEval is not implemented for this grid type.");
@@ -126,10 +133,12 @@ base::OperationMultipleEval* createOperationMultipleEval(
         throw base::factory_exception("HIP operator needs a HIPOperationConfiguration parameter");
       cfg = *hp;
     }
+    if (subspace) cfg.algorithm = datadriven::HIPAlgorithm::SUBSPACE;
     return new datadriven::OperationMultiEvalHIP(grid, dataset, cfg);
   }
   throw base::factory_exception(
-      "OperationMultiEval: this library provides the (STREAMING, HIP) operator only");
+      "OperationMultiEval: this library provides the (STREAMING, HIP) and (SUBSPACELINEAR, HIP) "
+      "operators only");
 }
 
 }  // namespace op_factory

@@ -1,0 +1,230 @@
+"""Parity of the subspace-indexed HIP operator (algorithm SUBSPACE, the
+SubspaceLinear semantics of OperationMultipleEvalSubspaceCombined) against
+the oracle, the reference's own SubspaceLinear operator (oracle/_ref) and the
+streaming HIP operator, through the C-ABI.
+
+Tolerances (relative, ||got - ref||_2 / ||ref||_2 unless stated): 1e-12 for
+mult and multTranspose (FP64 throughout). mult repeats bit for bit;
+multTranspose sums with FP64 atomic adds (in LDS, and in global memory for
+subspaces of more than 4096 cells), as the reference's SubspaceLinear
+multTranspose does with OpenMP atomics, so its repeats agree to 1e-14.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from sgpp_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    nb = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+def sub_op(capi, basis, X, lv=None, ix=None, **kw):
+    op = capi.MultiEval(basis, X, algorithm=capi.ALGO_SUBSPACE, **kw)
+    return op.prepare(lv, ix) if lv is not None else op
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("d,level,M", [(1, 6, 300), (2, 5, 5000), (5, 5, 10000), (8, 3, 4097),
+                                       (10, 3, 2000), (13, 2, 3000), (20, 2, 1500)])
+def test_parity_regular(gpu, basis, d, level, M):
+    capi = gpu
+    g = capi.Grid(basis, d).regular(level)
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(500 + d, M, d)
+    alpha = workloads.normal(workloads.stream(500 + d, 1), g.size)
+    y = workloads.normal(workloads.stream(500 + d, 2), M)
+    print("case basis=%d d=%d level=%d M=%d N=%d" % (basis, d, level, M, g.size), flush=True)
+    op = sub_op(capi, basis, X, lv, ix)
+    assert rel(op.mult(alpha), oracle.mult(basis, X, lv, ix, alpha)) <= 1e-12
+    assert rel(op.mult_transpose(y), oracle.mult_transpose(basis, X, lv, ix, y)) <= 1e-12
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("d,M", [(2, 3001), (3, 2500), (7, 2000), (12, 1500), (20, 800)])
+def test_parity_adaptive(gpu, basis, d, M):
+    """Adaptive grids: partly filled subspaces (table entries without a grid
+    point) and deep level vectors."""
+    capi = gpu
+    g = capi.Grid(basis, d).regular(3 if d <= 7 else 2)
+    for step in range(4 if d <= 7 else 3):
+        g.refine_surplus(1.0 + workloads.uniform(workloads.stream(600 + d, step), g.size),
+                         max(1, g.size // 6))
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(700 + d, M, d)
+    alpha = workloads.normal(workloads.stream(700 + d, 1), g.size)
+    y = workloads.normal(workloads.stream(700 + d, 2), M)
+    print("case basis=%d d=%d M=%d N=%d" % (basis, d, M, g.size), flush=True)
+    op = sub_op(capi, basis, X, lv, ix)
+    assert rel(op.mult(alpha), oracle.mult(basis, X, lv, ix, alpha)) <= 1e-12
+    assert rel(op.mult_transpose(y), oracle.mult_transpose(basis, X, lv, ix, y)) <= 1e-12
+
+
+def test_small_linear_workload_against_reference_subspace(gpu):
+    """Linear d=5 level 5 (N=1471), 10,000 uniform points against the
+    reference's OperationMultipleEvalSubspaceCombined and the definition."""
+    capi = gpu
+    c = workloads.config_small_linear()
+    lv, ix = c["grid"].level_index()
+    op = sub_op(capi, 0, c["X"], lv, ix)
+    r, a = op.mult(c["alpha"]), op.mult_transpose(c["y"])
+    assert rel(r, oracle.mult(0, c["X"], lv, ix, c["alpha"])) <= 1e-12
+    assert rel(a, oracle.mult_transpose(0, c["X"], lv, ix, c["y"])) <= 1e-12
+    if oracle.ref() is not None:
+        rop = oracle.RefOp(oracle.RefGrid(0, 5).regular(5), oracle.KIND_SUBSPACE, c["X"])
+        assert rel(r, rop.mult(c["alpha"])) <= 1e-12
+        assert rel(a, rop.mult_transpose(c["y"])) <= 1e-12
+
+
+def test_known_answers_and_boundaries(gpu):
+    capi = gpu
+    # base/tests/test_OperationMultipleEval.cpp:18-72
+    g = capi.Grid.linear(2).regular(2)
+    lv, ix = g.level_index()
+    op = sub_op(capi, 0, np.array([[0.5, 1.0], [0.3, 0.4], [0.9, 0.7]]), lv, ix)
+    np.testing.assert_allclose(op.mult(np.arange(1.0, 6.0)), [0.0, 2.72, 1.64], rtol=1e-12,
+                               atol=1e-14)
+    # base/tests/test_GridFactory.cpp:409-438
+    g = capi.Grid.linear(1).regular(2)
+    lv, ix = g.level_index()
+    op = sub_op(capi, 0, np.array([[0.25]]), lv, ix)
+    assert op.mult_transpose(np.array([1.0])).tolist() == [0.5, 1.0, 0.0]
+    assert op.mult(np.array([1.0, 0.0, 0.0])).tolist() == [0.5]
+    # coordinates 0 and 1 and cell boundaries: x = 1 lies in the last cell
+    # (where the ModLinear boundary function is 2), dyadic points on the
+    # edges between cells
+    rng = np.random.default_rng(3)
+    X = np.vstack([np.array([[0.0, 0.0], [1.0, 1.0], [0.5, 0.5], [0.25, 0.75], [0.125, 1.0]]),
+                   rng.integers(0, 33, size=(2000, 2)) / 32.0])
+    for basis in (0, 1):
+        g = capi.Grid(basis, 2).regular(5)
+        lv, ix = g.level_index()
+        alpha = np.linspace(-1, 1, g.size)
+        y = workloads.normal(workloads.stream(9, basis), X.shape[0])
+        op = sub_op(capi, basis, X, lv, ix)
+        r0 = oracle.mult(basis, X, lv, ix, alpha)
+        np.testing.assert_allclose(op.mult(alpha), r0, rtol=1e-13, atol=1e-14)
+        assert rel(op.mult_transpose(y), oracle.mult_transpose(basis, X, lv, ix, y)) <= 1e-13
+
+
+def test_against_streaming_operator_and_repeats(gpu):
+    """ModLinear d=6 level 4 with 20,000 points: the subspace operator equals
+    the streaming operator; mult repeats bit for bit, multTranspose to
+    rounding."""
+    capi = gpu
+    d = 6
+    g = capi.Grid.modlinear(d).regular(4)
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(81, 20000, d)
+    alpha = workloads.normal(workloads.stream(81, 1), g.size)
+    y = workloads.normal(workloads.stream(81, 2), X.shape[0])
+    op = sub_op(capi, 1, X, lv, ix)
+    st = capi.MultiEval(1, X).prepare(lv, ix)
+    r, a = op.mult(alpha), op.mult_transpose(y)
+    assert rel(r, st.mult(alpha)) <= 1e-12 and rel(a, st.mult_transpose(y)) <= 1e-12
+    assert (op.mult(alpha) == r).all()
+    assert rel(op.mult_transpose(y), a) <= 1e-14
+
+
+@pytest.mark.parametrize("basis", [0, 1])
+@pytest.mark.parametrize("d,level,M", [(1, 14, 5000), (2, 14, 1000)])
+def test_subspaces_larger_than_lds_chunks(gpu, basis, d, level, M):
+    """Subspaces of 8,192 cells (level 14 in one dimension, level sum 15 in
+    two), more than one multTranspose LDS chunk holds: those add with global
+    atomics into their segment's partial row."""
+    capi = gpu
+    g = capi.Grid(basis, d).regular(level)
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(800 + d, M, d)
+    alpha = workloads.normal(workloads.stream(800 + d, 1), g.size)
+    y = workloads.normal(workloads.stream(800 + d, 2), M)
+    print("case basis=%d d=%d level=%d M=%d N=%d" % (basis, d, level, M, g.size), flush=True)
+    op = sub_op(capi, basis, X, lv, ix)
+    a = op.mult_transpose(y)
+    assert rel(op.mult(alpha), oracle.mult(basis, X, lv, ix, alpha)) <= 1e-12
+    assert rel(a, oracle.mult_transpose(basis, X, lv, ix, y)) <= 1e-12
+    assert rel(op.mult_transpose(y), a) <= 1e-14
+
+
+def test_factory_branch_and_refinement(gpu):
+    """The operator bound to a grid through the factory's SUBSPACELINEAR
+    branch; prepare() re-reads the grid after refinement (new subspaces,
+    partly filled tables)."""
+    capi = gpu
+    g = capi.Grid.linear(4).regular(3)
+    X = workloads.uniform_points(91, 3000, 4)
+    op = capi.MultiEval(0, X, grid=g, algorithm=capi.ALGO_SUBSPACE).prepare()
+    for step in range(3):
+        lv, ix = g.level_index()
+        alpha = workloads.normal(workloads.stream(92, step), g.size)
+        y = workloads.normal(workloads.stream(93, step), X.shape[0])
+        assert rel(op.mult(alpha), oracle.mult(0, X, lv, ix, alpha)) <= 1e-12
+        assert rel(op.mult_transpose(y), oracle.mult_transpose(0, X, lv, ix, y)) <= 1e-12
+        g.refine_surplus(np.abs(alpha), 10)
+        op.prepare()
+
+
+def test_counts_and_errors(gpu):
+    capi = gpu
+    g = capi.Grid.linear(3).regular(4)
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(95, 1000, 3)
+    op = sub_op(capi, 0, X, lv, ix)
+    w = op.count_work(capi.PREC_FP64)
+    # regular level 4 in 3 dimensions: the level vectors with sum <= 6, 20 subspaces
+    assert w["mult_pairs"] % 20 == 0 and w["mult_pairs"] // 20 >= X.shape[0]
+    assert w["mult_pairs"] == w["multT_pairs"] and w["mult_hat_steps"] == w["multT_hat_steps"]
+    assert w["mult_pairs"] <= w["mult_hat_steps"] <= 3 * w["mult_pairs"]
+    # FP64 only, d <= 20, no no-allocation variant
+    import torch
+    da = torch.zeros(g.size, dtype=torch.float64, device="cuda:0")
+    dr = torch.empty(X.shape[0], dtype=torch.float64, device="cuda:0")
+    torch.cuda.synchronize()
+    with pytest.raises(capi.SGppHIPError) as e:
+        capi.check(capi.F["sgpp_hip_meval_mult_device_noalloc"](op.h, da.data_ptr(),
+                                                                 dr.data_ptr(), capi.PREC_FP64))
+    assert e.value.status_name == "INVALID_ARGUMENT"
+    with pytest.raises(capi.SGppHIPError) as e:
+        sub_op(capi, 0, X, precision=capi.PREC_FP32)
+    assert e.value.status_name == "INVALID_ARGUMENT"
+    with pytest.raises(capi.SGppHIPError) as e:
+        sub_op(capi, 0, workloads.uniform_points(96, 100, 21))
+    assert e.value.status_name == "INVALID_ARGUMENT"
+    with pytest.raises(capi.SGppHIPError):
+        sub_op(capi, 0, X).mult_transpose(np.ones(X.shape[0]))  # no grid prepared
+    with pytest.raises(capi.SGppHIPError):
+        sub_op(capi, 0, X, np.zeros((3, 3)), np.ones((3, 3)))  # level 0
+
+
+def test_full_size_linear_d8_level6(gpu):
+    """Linear d=8 level 6 (N = 31,745, 1,287 subspaces) with the 500,000-point
+    regression dataset: the subspace operator against the streaming operator
+    (an independent algorithm, itself checked against the oracle), the
+    adjoint identity, and sampled rows and columns against the oracle."""
+    capi = gpu
+    c = workloads.config_regression()
+    lv, ix = c["grid"].level_index()
+    X = c["X"]
+    M, N = X.shape[0], lv.shape[0]
+    assert (M, N) == (500000, 31745)
+    alpha = workloads.normal(workloads.stream(97, 1), N)
+    y = c["y"]
+    op = sub_op(capi, 0, X, lv, ix)
+    st = capi.MultiEval(0, X).prepare(lv, ix)
+    r, a = op.mult(alpha), op.mult_transpose(y)
+    assert rel(r, st.mult(alpha)) <= 1e-12 and rel(a, st.mult_transpose(y)) <= 1e-12
+    assert abs(np.dot(r, y) - np.dot(alpha, a)) <= 1e-12 * np.linalg.norm(r) * np.linalg.norm(y)
+    rows = np.arange(0, M, 997)
+    assert rel(r[rows], oracle.mult(0, X[rows], lv, ix, alpha)) <= 1e-12
+    cols = np.arange(0, N, 211)
+    # column j of B^T y = sum_m phi_j(x_m) y_m, from the oracle's B^T on all data
+    # restricted to the sampled grid points
+    a0 = oracle.mult_transpose(0, X, lv[cols], ix[cols], y)
+    assert rel(a[cols], a0) <= 1e-12
+    counts = op.count_work(capi.PREC_FP64)
+    assert counts["mult_pairs"] // 1287 >= M
+
