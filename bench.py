@@ -432,6 +432,18 @@ def solve_bench(args):
             info["alpha_rel_err_vs_cpu_fp64"] = float(np.linalg.norm(alpha - ref["alpha"]) /
                                                       np.linalg.norm(ref["alpha"]))
         runs[name] = info
+    # the subspace-indexed operator (SubspaceLinear semantics, FP64): kernels
+    # and the same FP64 CG solve
+    sop = capi.MultiEval(capi.BASIS_LINEAR, X, algorithm=capi.ALGO_SUBSPACE).prepare(lv, ix)
+    step_s, mult_ms, multT_ms = timed_steps(sop, capi.PREC_FP64, d_a, d_y, d_r, d_b, 5, 1, torch)
+    kern["subspace_fp64"] = {"pair_evals_per_s": 2.0 * M * N / step_s, "mult_ms": mult_ms,
+                             "multT_ms": multT_ms, "executed_work": sop.count_work(capi.PREC_FP64)}
+    alpha_s, info_s = sop.solve(y, lam, max_iterations=10000, epsilon=1e-10, mode=capi.SOLVE_FP64)
+    if ref is not None:
+        info_s["alpha_rel_err_vs_cpu_fp64"] = float(np.linalg.norm(alpha_s - ref["alpha"]) /
+                                                    np.linalg.norm(ref["alpha"]))
+    runs["fp64_cg_subspace"] = info_s
+    del sop
     line = {
         "metric": "regression solve time to a 1e-10 relative residual, FP32+FP64 refinement vs FP64",
         "value": runs["fp64_cg"]["seconds"] / runs["mixed_refinement"]["seconds"],
@@ -444,6 +456,7 @@ def solve_bench(args):
         "kernel_speedup_fp32_vs_fp64": kern["fp32"]["pair_evals_per_s"] / kern["fp64"]["pair_evals_per_s"],
         "fp64_cg": runs["fp64_cg"],
         "mixed_refinement": runs["mixed_refinement"],
+        "fp64_cg_subspace_operator": runs["fp64_cg_subspace"],
         "cpu_reference": ({"iterations": int(ref["iterations"]), "seconds": float(ref["seconds"]),
                            "threads": int(ref["threads"]),
                            "solver": "reference ConjugateGradients on OperationMultipleEvalSubspaceCombined",
@@ -701,6 +714,20 @@ def main():
                                          "support culling skips as exact zeros"),
         "setup_s": setup_s,
     }
+    # the subspace-indexed operator (FP64) on the same workload
+    sop = capi.MultiEval(capi.BASIS_MODLINEAR, X, algorithm=capi.ALGO_SUBSPACE).prepare(lv, ix)
+    sstep, smult, smultT = timed_steps(sop, capi.PREC_FP64, d_alpha, d_y, d_r, d_a, args.steps,
+                                       args.warmup, torch)
+    rs, as_ = d_r.cpu().numpy(), d_a.cpu().numpy()
+    line["subspace_fp64"] = {
+        "value": 2.0 * pairs / sstep, "unit": "pair-evals/s", "ms_per_step": sstep * 1e3,
+        "mult_ms": smult, "multT_ms": smultT, "speedup_vs_streaming_fp64": res["fp64"]["step_s"] / sstep,
+        "executed_work": sop.count_work(capi.PREC_FP64),
+        "rel_diff_vs_streaming_fp64": {
+            "mult": float(np.linalg.norm(rs - outs["fp64"][0]) / np.linalg.norm(outs["fp64"][0])),
+            "multT": float(np.linalg.norm(as_ - outs["fp64"][1]) / np.linalg.norm(outs["fp64"][1]))},
+    }
+    del sop
     if not args.no_check:
         line["rel_err"] = accuracy(capi.BASIS_MODLINEAR, X, lv, ix, alpha, y, outs)
     if not args.no_cpu:

@@ -201,12 +201,11 @@ class DeviceMultiEval {
 
   // subspace algorithm: subspace records (sorted by level vector), the index
   // tables (grid point per cell, -1 where the grid has none), each grid
-  // point's table entry, and a table-shaped FP64 buffer (weights for mult,
-  // sums for multTranspose); sub_recompute_ = sum over subspaces of the
-  // dimensions not shared with the previous one
+  // point's table entry, and a table-shaped FP64 buffer (mult weights);
+  // sub_recompute_(t_) = dimensions mult (multTranspose) evaluates per data row
   int nsub_ = 0;
   int64_t ntable_ = 0;
-  uint64_t sub_recompute_ = 0;
+  uint64_t sub_recompute_ = 0, sub_recompute_t_ = 0;
   void* sub_desc_ = nullptr;  size_t sub_desc_cap_ = 0;
   void* sub_ti_ = nullptr;    size_t sub_ti_cap_ = 0;
   void* sub_te_ = nullptr;    size_t sub_te_cap_ = 0;

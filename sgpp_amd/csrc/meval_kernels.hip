@@ -568,10 +568,11 @@ void DeviceMultiEval::count_work(Precision p, uint64_t out[4]) {
   if (N_ == 0) throw input_error("grid not set (call prepare)");
   if (!serves(p)) throw input_error("precision not enabled for this instance");
   if (algorithm_ == Algorithm::Subspace) {
-    // every data row (padding included) visits every subspace and recomputes
+    // every data row (padding included) visits every subspace and evaluates
     // the dimensions it does not share with the previous one
     out[1] = out[3] = static_cast<uint64_t>(Mpad_) * static_cast<uint64_t>(nsub_);
-    out[0] = out[2] = static_cast<uint64_t>(Mpad_) * sub_recompute_;
+    out[0] = static_cast<uint64_t>(Mpad_) * sub_recompute_;
+    out[2] = static_cast<uint64_t>(Mpad_) * sub_recompute_t_;
     return;
   }
   check_hip(hipSetDevice(device_), "hipSetDevice");

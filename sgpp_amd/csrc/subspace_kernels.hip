@@ -63,6 +63,9 @@ static_assert(sizeof(SubspaceDesc) == 32, "one subspace record is 32 bytes");
 
 // LDS table entries of one multTranspose chunk (32 KiB of FP64)
 constexpr int kSubLds = 4096;
+// Subspace records the kernels load at once (one wait on the scalar loads per
+// batch); the record array carries kSubBatch - 1 records of padding.
+constexpr int kSubBatch = 4;
 
 // A multTranspose chunk: subspaces [s0, s1), table entries [off0, off0 + n);
 // big: one subspace whose table exceeds kSubLds (global atomics)
@@ -75,23 +78,51 @@ struct alignas(16) SubChunk {
 static_assert(sizeof(SubChunk) == 32, "one chunk record is 32 bytes");
 
 // Per dimension: the data point's cell in a level-l subspace and the factor
-// of its basis function there. Linear: 1 - |2^l x - i|. ModLinear: level 1
-// is constant 1; i = 1 and i = 2^l - 1 are the boundary-modified functions
-// max(0, 2 - 2^l x) and max(0, 2^l x - i + 1) (LinearModifiedBasis.hpp:36-53).
-// x = 1 would fall into cell 2^l; it belongs to the last cell (index
-// 2^l - 1), where the Linear factor is 0 and the ModLinear boundary factor 2.
+// of its basis function there. With t = 2^(l-1) x, the support of the
+// function with index i = 2h + 1 is [h, h + 1) in t: h = floor(t) (the table
+// position), u = t - h in [0, 1), and
+//   Linear hat:            min(2u, 2 - 2u)  (1 - |2^l x - i|, exact in FP64)
+//   ModLinear, level 1:    1
+//              h = 0:      2 - 2u           (max(0, 2 - 2^l x) on its cell)
+//              h = last:   2u               (max(0, 2^l x - i + 1))
+//              otherwise:  the hat          (LinearModifiedBasis.hpp:36-53).
+// x = 1 belongs to the last cell (h = 2^(l-1) - 1, u = 1), where the hat is 0
+// and the ModLinear boundary function 2.
 template <bool kMod>
 __device__ __forceinline__ double cell_factor(double x, int l, unsigned& half_index) {
-  const double t = __builtin_ldexp(x, l);  // 2^l x, exact
+  const double t = __builtin_ldexp(x, l - 1);  // exact
+  const unsigned last = (1u << (l - 1)) - 1u;
   const unsigned c = static_cast<unsigned>(t);
-  const unsigned top = (1u << l) - 1u;
-  const unsigned i = (c | 1u) < top ? (c | 1u) : top;
-  half_index = i >> 1;
-  if (!kMod) return 1.0 - __builtin_fabs(t - static_cast<double>(i));
+  const unsigned h = c < last ? c : last;
+  half_index = h;
+  const double u = t - static_cast<double>(h);  // exact
+  const double a = u + u;
+  const double b = 2.0 - a;
+  if (!kMod) return __builtin_fmin(a, b);
   if (l == 1) return 1.0;
-  if (i == 1u) return __builtin_fmax(0.0, 2.0 - t);
-  if (i == top) return __builtin_fmax(0.0, t - static_cast<double>(i) + 1.0);
-  return 1.0 - __builtin_fabs(t - static_cast<double>(i));
+  return h == 0u ? b : (h == last ? a : __builtin_fmin(a, b));
+}
+
+// Dimensions [recalc, D) of subspace sd; those below recalc are shared with
+// the previous subspace.
+template <int D, bool kMod>
+__device__ __forceinline__ void walk_subspace(const double (&x)[D], const SubspaceDesc& sd,
+                                              int recalc, double (&prod)[D + 1],
+                                              unsigned (&flat)[D + 1]) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (d < recalc) continue;
+    const int l = sd.level[d];
+    if (kMod && l == 1) {
+      prod[d + 1] = prod[d];
+      flat[d + 1] = flat[d];
+      continue;
+    }
+    unsigned h;
+    const double f = cell_factor<kMod>(x[d], l, h);
+    prod[d + 1] = prod[d] * f;
+    flat[d + 1] = (flat[d] << (l - 1)) + h;
+  }
 }
 
 // mult: lanes own data points X[:, m] (stored order), out[m] = r(x_m).
@@ -108,19 +139,35 @@ __global__ __launch_bounds__(kSubBlock) void subspace_mult_kernel(
   prod[0] = 1.0;
   flat[0] = 0u;
   double acc = 0.0;
-  for (int s = 0; s < S; ++s) {
-    const SubspaceDesc sd = desc[s];
+  // a batch's weights are gathered before any is used, and used after the
+  // next batch's gathers are issued (the gathers' latency overlaps the
+  // next batch's products)
+  double wp[kSubBatch], pp[kSubBatch];
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      if (d < sd.recalc) continue;  // shared with the previous subspace
-      const int l = sd.level[d];
-      unsigned h;
-      const double f = cell_factor<kMod>(x[d], l, h);
-      prod[d + 1] = prod[d] * f;
-      flat[d + 1] = (flat[d] << (l - 1)) + h;
+  for (int k = 0; k < kSubBatch; ++k) wp[k] = pp[k] = 0.0;
+  for (int s = 0; s < S; s += kSubBatch) {
+    SubspaceDesc sd[kSubBatch];
+#pragma unroll
+    for (int k = 0; k < kSubBatch; ++k) sd[k] = desc[s + k];
+    double wn[kSubBatch], pn[kSubBatch];
+#pragma unroll
+    for (int k = 0; k < kSubBatch; ++k) {
+      wn[k] = pn[k] = 0.0;
+      if (s + k < S) {
+        walk_subspace<D, kMod>(x, sd[k], sd[k].recalc, prod, flat);
+        wn[k] = TW[sd[k].off + flat[D]];
+        pn[k] = prod[D];
+      }
     }
-    acc = __builtin_fma(TW[sd.off + flat[D]], prod[D], acc);
+#pragma unroll
+    for (int k = 0; k < kSubBatch; ++k) {
+      acc = __builtin_fma(wp[k], pp[k], acc);
+      wp[k] = wn[k];
+      pp[k] = pn[k];
+    }
   }
+#pragma unroll
+  for (int k = 0; k < kSubBatch; ++k) acc = __builtin_fma(wp[k], pp[k], acc);
   out[m] = acc;
 }
 
@@ -156,18 +203,17 @@ __global__ __launch_bounds__(kSubBlock) void subspace_multT_kernel(
     unsigned flat[D + 1];
     prod[0] = 1.0;
     flat[0] = 0u;
-    for (int s = ch.s0; s < ch.s1; ++s) {
-      const SubspaceDesc sd = desc[s];
-      const int recalc = s == ch.s0 ? 0 : sd.recalc;  // a chunk starts from scratch
+    for (int sb = ch.s0; sb < ch.s1; sb += kSubBatch) {
+      SubspaceDesc sdb[kSubBatch];
 #pragma unroll
-      for (int d = 0; d < D; ++d) {
-        if (d < recalc) continue;  // shared with the previous subspace
-        const int l = sd.level[d];
-        unsigned h;
-        const double f = cell_factor<kMod>(x[d], l, h);
-        prod[d + 1] = prod[d] * f;
-        flat[d + 1] = (flat[d] << (l - 1)) + h;
-      }
+      for (int k = 0; k < kSubBatch; ++k) sdb[k] = desc[sb + k];
+#pragma unroll
+      for (int k = 0; k < kSubBatch; ++k) {
+      const int s = sb + k;
+      if (s >= ch.s1) break;
+      const SubspaceDesc& sd = sdb[k];
+      // a chunk starts from scratch
+      walk_subspace<D, kMod>(x, sd, s == ch.s0 ? 0 : sd.recalc, prod, flat);
       const double v = y * prod[D];
       const unsigned e = static_cast<unsigned>(sd.off - ch.off0) + flat[D];
       const unsigned e0 =
@@ -189,6 +235,7 @@ __global__ __launch_bounds__(kSubBlock) void subspace_multT_kernel(
           atomicAdd(&prow[e], v);
         else
           atomicAdd(&acc[e], v);
+      }
       }
     }
   }
@@ -361,8 +408,20 @@ void DeviceMultiEval::set_grid_subspace(const uint32_t* level, const uint32_t* i
   nchunk_ = static_cast<int>(chunks.size());
   nsub_ = static_cast<int>(desc.size());
   ntable_ = total;
-  sub_recompute_ = 0;
-  for (const SubspaceDesc& sd : desc) sub_recompute_ += static_cast<uint64_t>(D_ - sd.recalc);
+  // evaluated dimensions per data row: those not shared with the previous
+  // subspace (multTranspose: nor at a chunk's start), ModLinear level 1 excluded
+  const bool mod = basis_ == Basis::ModLinear;
+  auto evaluated = [&](const SubspaceDesc& sd, int from) {
+    uint64_t n = 0;
+    for (int d = from; d < static_cast<int>(D_); ++d) n += (mod && sd.level[d] == 1) ? 0 : 1;
+    return n;
+  };
+  sub_recompute_ = sub_recompute_t_ = 0;
+  for (const SubspaceDesc& sd : desc) sub_recompute_ += evaluated(sd, sd.recalc);
+  for (const SubChunk& c : chunks)
+    for (int q = c.s0; q < c.s1; ++q)
+      sub_recompute_t_ += evaluated(desc[q], q == c.s0 ? 0 : desc[q].recalc);
+  desc.resize(desc.size() + kSubBatch - 1, SubspaceDesc{});  // batch padding
   ensure(&sub_desc_, &sub_desc_cap_, desc.size() * sizeof(SubspaceDesc));
   check_hip(hipMemcpy(sub_desc_, desc.data(), desc.size() * sizeof(SubspaceDesc),
                       hipMemcpyHostToDevice),
