@@ -1,12 +1,15 @@
 """Grid engine (input builder): sequence numbers, leaf flags and surpluses
 must equal SG++'s bit for bit. CPU only."""
 import math
+import os
 
 import numpy as np
 import pytest
 
 import oracle
 from sgpp_amd import capi, workloads
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 ref_only = pytest.mark.skipif(oracle.ref() is None, reason="reference build not available")
 
@@ -171,3 +174,28 @@ def test_modlinear_d10_refinement_matches_reference():
     rlv, rix = r.level_index()
     assert g.size == r.size
     assert (lv == rlv).all() and (ix == rix).all()
+
+
+def grid_digest(g):
+    """SHA-256 of the level then index arrays (uint32 little-endian, N x d in
+    sequence order), as tests/golden/make_grid_hashes.py writes them for the
+    reference's own refinement loop."""
+    import hashlib
+    lv, ix = g.level_index()
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(lv, dtype="<u4").tobytes())
+    h.update(np.ascontiguousarray(ix, dtype="<u4").tobytes())
+    return h.hexdigest()
+
+
+def test_modlinear_d10_benchmark_grid_matches_reference_hash():
+    """The full 100,015-point ModLinear d=10 grid of the benchmark, point for
+    point (order included) the grid the reference's own Hierarchisation +
+    SurplusRefinementFunctor loop builds (fixture from oracle/_ref)."""
+    import json
+    with open(os.path.join(GOLDEN, "grid_hashes.json")) as f:
+        ref = json.load(f)["modlinear_d10_l4_gaussian_bump_100k"]
+    g, _, steps = workloads.adaptive_grid(capi.BASIS_MODLINEAR, 10, 4, workloads.gaussian_bump,
+                                          200, 100000)
+    assert (g.size, steps) == (ref["points"], ref["refinement_steps"])
+    assert grid_digest(g) == ref["sha256"]

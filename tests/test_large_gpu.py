@@ -8,12 +8,19 @@ refinement of f(x) = prod_j 4 x_j (1 - x_j)(1 + 0.5 cos(pi x_j)). The
 workload's stated f = prod_j (1 + 0.5 cos(pi x_j)) is not zero on the
 boundary of the cube, which a grid without boundary points cannot represent:
 surplus refinement then keeps halving the cells next to the faces until the
-levels run out. The damping factor 4 x (1 - x) makes f vanish there.
+levels run out. The damping factor 4 x (1 - x) makes f vanish there. The grid
+is point for point (order included) the one the reference's own
+Hierarchisation + SurplusRefinementFunctor loop builds
+(tests/golden/grid_hashes.json, written from oracle/_ref).
 
 Checks on the first 100,000 data points: FP32 against FP64 GPU products
 (1e-5 relative, both operators), and FP64 against the oracle on 64 sampled
 rows (mult) and 16 sampled grid points (multTranspose), 1e-12 relative. The
 solve itself: 5 iterations, finite alpha, residual reduced."""
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -36,6 +43,14 @@ def test_large_streaming_workload(gpu):
     print("grid %d points after %d refinement steps, %d data points" % (N, c["steps"], M),
           flush=True)
     assert N >= 1000000 and M == 10000000 and lv.shape[1] == 20
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                           "grid_hashes.json")) as f:
+        ref = json.load(f)["linear_d20_l4_damped_cosine_product_1m"]
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(lv, dtype="<u4").tobytes())
+    h.update(np.ascontiguousarray(ix, dtype="<u4").tobytes())
+    assert (N, c["steps"], h.hexdigest()) == (ref["points"], ref["refinement_steps"],
+                                             ref["sha256"])
     assert np.array_equal(X, X.astype(np.float32).astype(np.float64))
     op = capi.MultiEval(capi.BASIS_LINEAR, X, precision=capi.PREC_FP32).prepare(lv, ix)
     alpha, info = op.solve(y, lam, mode=capi.SOLVE_FP32, max_iterations=5, epsilon=0.0)
