@@ -92,6 +92,8 @@ typedef struct sgpp_hip_grid_desc {
   size_t dim;
 } sgpp_hip_grid_desc;
 
+/* Callers compiled against a header without the `algorithm` member must be
+ * rebuilt: the library reads every member below. */
 typedef struct sgpp_hip_params {
   int precision;        /* SGPP_HIP_PREC_* used by mult / mult_transpose */
   int keep_fp64;        /* also hold FP64 data when precision is FP32 */
@@ -135,6 +137,14 @@ sgpp_hip_status sgpp_hip_meval_mult(sgpp_hip_meval* h, const double* alpha, doub
 /* multTranspose (OperationMultipleEval.hpp:81): source M -> result N */
 sgpp_hip_status sgpp_hip_meval_mult_transpose(sgpp_hip_meval* h, const double* source,
                                               double* result);
+/* range forms (OperationMultipleEval.hpp:70-73, 97-100): rows [start, end) of
+ * B alpha into result (end - start values), entries [start, end) of B^T source
+ * into result (end - start values); start > end or end past the data / grid
+ * size is SGPP_HIP_ERR_INVALID_ARGUMENT */
+sgpp_hip_status sgpp_hip_meval_mult_range(sgpp_hip_meval* h, const double* alpha, size_t start,
+                                          size_t end, double* result);
+sgpp_hip_status sgpp_hip_meval_mult_transpose_range(sgpp_hip_meval* h, const double* source,
+                                                    size_t start, size_t end, double* result);
 /* same on device pointers, asynchronous on the operator's stream */
 sgpp_hip_status sgpp_hip_meval_mult_device(sgpp_hip_meval* h, const double* d_alpha,
                                            double* d_result, int precision);

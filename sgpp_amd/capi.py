@@ -149,6 +149,9 @@ SIGNATURES = {
     "sgpp_hip_meval_prepare_grid": (_st, [_p]),
     "sgpp_hip_meval_mult": (_st, [_p, _dp, _dp]),
     "sgpp_hip_meval_mult_transpose": (_st, [_p, _dp, _dp]),
+    "sgpp_hip_meval_mult_range": (_st, [_p, _dp, ctypes.c_size_t, ctypes.c_size_t, _dp]),
+    "sgpp_hip_meval_mult_transpose_range": (_st, [_p, _dp, ctypes.c_size_t, ctypes.c_size_t,
+                                                  _dp]),
     "sgpp_hip_meval_mult_device": (_st, [_p, _p, _p, ctypes.c_int]),
     "sgpp_hip_meval_mult_transpose_device": (_st, [_p, _p, _p, ctypes.c_int]),
     "sgpp_hip_meval_synchronize": (_st, [_p]),
@@ -393,6 +396,24 @@ class MultiEval:
         a = np.empty(self.N, dtype=np.float64)
         check(F["sgpp_hip_meval_mult_transpose"](self.h, _dptr(y), _dptr(a)))
         return a
+
+    def mult_range(self, alpha, start, end):
+        """rows [start, end) of B alpha"""
+        a = _f64(alpha)
+        if a.size != self.N:
+            raise ValueError("alpha must have N entries")
+        r = np.empty(max(0, end - start), dtype=np.float64)
+        check(F["sgpp_hip_meval_mult_range"](self.h, _dptr(a), start, end, _dptr(r)))
+        return r
+
+    def mult_transpose_range(self, source, start, end):
+        """entries [start, end) of B^T source"""
+        y = _f64(source)
+        if y.size != self.M:
+            raise ValueError("source must have M entries")
+        r = np.empty(max(0, end - start), dtype=np.float64)
+        check(F["sgpp_hip_meval_mult_transpose_range"](self.h, _dptr(y), start, end, _dptr(r)))
+        return r
 
     def mult_device(self, d_alpha_ptr, d_result_ptr, precision=None):
         p = self.precision if precision is None else precision

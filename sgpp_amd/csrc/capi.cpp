@@ -392,6 +392,50 @@ sgpp_hip_status sgpp_hip_meval_mult_transpose(sgpp_hip_meval* h, const double* s
   });
 }
 
+sgpp_hip_status sgpp_hip_meval_mult_range(sgpp_hip_meval* h, const double* alpha, size_t start,
+                                          size_t end, double* result) {
+  return guarded([&] {
+    REQUIRE(h && alpha && (result || start == end), "null argument");
+    const size_t M = h->dev->num_data();
+    REQUIRE(start <= end && end <= M, "invalid data range");
+    DataVector r;
+    if (h->op) {
+      DataVector a(alpha, h->dev->num_grid());
+      h->op->mult(a, r, start, end);
+      h->duration = h->op->getDuration();
+    } else {
+      DataVector full(M);
+      h->dev->mult_host(alpha, full.data(), h->prec);
+      h->duration = h->dev->last_wall_s();
+      r.assign(full.begin() + start, full.begin() + end);
+    }
+    if (end > start) std::memcpy(result, r.data(), (end - start) * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_meval_mult_transpose_range(sgpp_hip_meval* h, const double* source,
+                                                    size_t start, size_t end, double* result) {
+  return guarded([&] {
+    REQUIRE(h && source && (result || start == end), "null argument");
+    const size_t N = h->dev->num_grid();
+    REQUIRE(start <= end && end <= N, "invalid grid range");
+    DataVector r;
+    if (h->op) {
+      DataVector y(source, h->dev->num_data());
+      h->op->multTranspose(y, r, start, end);
+      h->duration = h->op->getDuration();
+    } else {
+      DataVector full(N);
+      h->dev->mult_transpose_host(source, full.data(), h->prec);
+      h->duration = h->dev->last_wall_s();
+      r.assign(full.begin() + start, full.begin() + end);
+    }
+    if (end > start) std::memcpy(result, r.data(), (end - start) * sizeof(double));
+    return SGPP_HIP_OK;
+  });
+}
+
 sgpp_hip_status sgpp_hip_meval_mult_device(sgpp_hip_meval* h, const double* d_alpha,
                                            double* d_result, int precision) {
   return guarded([&] {

@@ -32,6 +32,15 @@ class OperationMultiEvalHIP : public base::OperationMultipleEval {
 
   void mult(base::DataVector& alpha, base::DataVector& result) override;
   void multTranspose(base::DataVector& source, base::DataVector& result) override;
+  // Range forms (OperationMultipleEval.hpp:70-73, 97-100) with the semantics of
+  // the streaming OCL operator (OperationMultiEvalStreamingOCLMultiPlatform.hpp:178-242):
+  // result is resized to end - start and holds rows [start, end) of B alpha
+  // (entries [start, end) of B^T source). The full product is computed and the
+  // range kept: one GPU evaluates the whole operator faster than a partition.
+  void mult(base::DataVector& alpha, base::DataVector& result, size_t startIndexData,
+            size_t endIndexData) override;
+  void multTranspose(base::DataVector& source, base::DataVector& result, size_t startIndexGrid,
+                     size_t endIndexGrid) override;
   void prepare() override;
   double getDuration() override { return duration_; }
   std::string getImplementationName() override;

@@ -79,6 +79,26 @@ void OperationMultiEvalHIP::multTranspose(base::DataVector& source, base::DataVe
   duration_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+void OperationMultiEvalHIP::mult(base::DataVector& alpha, base::DataVector& result,
+                                 size_t startIndexData, size_t endIndexData) {
+  if (startIndexData > endIndexData || endIndexData > numData_)
+    throw base::data_exception("mult: invalid data range");
+  base::DataVector full(numData_);
+  mult(alpha, full);
+  result.resize(endIndexData - startIndexData);
+  for (size_t i = startIndexData; i < endIndexData; ++i) result[i - startIndexData] = full[i];
+}
+
+void OperationMultiEvalHIP::multTranspose(base::DataVector& source, base::DataVector& result,
+                                          size_t startIndexGrid, size_t endIndexGrid) {
+  if (startIndexGrid > endIndexGrid || endIndexGrid > numGrid_)
+    throw base::data_exception("multTranspose: invalid grid range");
+  base::DataVector full(numGrid_);
+  multTranspose(source, full);
+  result.resize(endIndexGrid - startIndexGrid);
+  for (size_t i = startIndexGrid; i < endIndexGrid; ++i) result[i - startIndexGrid] = full[i];
+}
+
 void OperationMultiEvalHIP::multDevice(const double* a, double* r) {
   dev_->mult_device(a, r, to_dev(config_.precision));
 }

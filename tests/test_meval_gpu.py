@@ -333,3 +333,34 @@ def test_full_size_properties(gpu):
     t0 = oracle.mult_transpose(1, X, lv[cols], ix[cols], c["y"])
     assert rel(r64.cpu().numpy()[rows], r0) <= 1e-12
     assert rel(t64.cpu().numpy()[cols], t0) <= 1e-12
+
+
+def test_range_forms(gpu):
+    """mult / multTranspose over a range of data rows / grid points
+    (OperationMultipleEval.hpp:70-73, 97-100): the slice of the full product,
+    for an operator from the factory and for a plain handle, both precisions
+    of the streaming kernels and the subspace kernels; empty and invalid
+    ranges."""
+    capi = gpu
+    g = capi.Grid.modlinear(3).regular(4)
+    lv, ix = g.level_index()
+    X = workloads.uniform_points(61, 3000, 3)
+    alpha = workloads.normal(workloads.stream(61, 1), g.size)
+    y = workloads.normal(workloads.stream(61, 2), X.shape[0])
+    ops = [capi.MultiEval(1, X, grid=g).prepare(),
+           capi.MultiEval(1, X, precision=1).prepare(lv, ix),
+           capi.MultiEval(1, X, algorithm=capi.ALGO_SUBSPACE).prepare(lv, ix)]
+    for op in ops:
+        r, a = op.mult(alpha), op.mult_transpose(y)
+        assert np.array_equal(op.mult_range(alpha, 100, 2100), r[100:2100])
+        assert np.array_equal(op.mult_range(alpha, 0, X.shape[0]), r)
+        # (the subspace multTranspose sums with atomics: equal to rounding)
+        assert rel(op.mult_transpose_range(y, 7, g.size), a[7:]) <= 1e-14
+        assert op.mult_range(alpha, 5, 5).size == 0
+        for bad in ((10, 5), (0, X.shape[0] + 1)):
+            with pytest.raises(capi.SGppHIPError) as e:
+                op.mult_range(alpha, *bad)
+            assert e.value.status_name == "INVALID_ARGUMENT"
+        with pytest.raises(capi.SGppHIPError) as e:
+            op.mult_transpose_range(y, 0, g.size + 1)
+        assert e.value.status_name == "INVALID_ARGUMENT"
