@@ -248,6 +248,16 @@ __device__ __forceinline__ int eval_group(const T* __restrict__ tile, const int 
 #pragma unroll
       for (int dd = 0; dd < DB; ++dd)
         if (d0 + dd < D) li[q][dd] = rows[jj[q] * D + d0 + dd];
+    if constexpr (kSkip) {
+      // every pair of the batch is used here, so the loads are issued together
+      // and waited for once (left to itself the compiler sinks each load into
+      // the branch of its dimension's mask bit: one wait per branch)
+#pragma unroll
+      for (int q = 0; q < J; ++q)
+#pragma unroll
+        for (int dd = 0; dd < DB; ++dd)
+          if (d0 + dd < D) asm volatile("" ::"s"(li[q][dd].L), "s"(li[q][dd].I));
+    }
     const uint32_t live = kSkip ? (live_all >> d0) : ~0u;
     evaluated += __builtin_popcount(live & ((1u << (D - d0 < DB ? D - d0 : DB)) - 1u));
 #pragma unroll
