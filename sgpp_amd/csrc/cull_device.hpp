@@ -159,22 +159,52 @@ __device__ __forceinline__ bool tile_may_hit(const float* __restrict__ TB, int64
   return h != 0u;
 }
 
+// The same pre-test for up to 64 tiles at once, one tile per lane: bit t of
+// the result is set when tile t0 + t (t < nt <= 64) may hit the box. One
+// vector load round trip for a whole window of tiles instead of one scalar
+// round trip per tile.
+template <int DC, int D, typename T>
+__device__ __forceinline__ unsigned long long tiles_may_hit(const float* __restrict__ TB,
+                                                            int64_t t0, int nt, int lane,
+                                                            const WaveBox<DC, T>& box) {
+  unsigned h = 0u;
+  if (lane < nt) {
+    const float* tb = TB + (t0 + lane) * D * 2;
+    h = 1u;
+#pragma unroll
+    for (int d0 = 0; d0 < DC; d0 += kTestBatch) {
+      float b[2 * kTestBatch];
+#pragma unroll
+      for (int e = 0; e < 2 * kTestBatch; ++e)
+        if (2 * d0 + e < 2 * DC) b[e] = tb[2 * d0 + e];
+#pragma unroll
+      for (int dd = 0; dd < kTestBatch; ++dd)
+        if (d0 + dd < DC)
+          h &= static_cast<unsigned>(static_cast<T>(b[2 * dd]) < box.h[d0 + dd]) &
+               static_cast<unsigned>(static_cast<T>(b[2 * dd + 1]) > box.l[d0 + dd]);
+    }
+  }
+  return __ballot(h != 0u);
+}
+
 // One lane per point of the tile: which points may be nonzero in the box
 // (the lane's (L, I) row is read in batches of kTestBatch dimensions).
 // Rows hold D dimensions; the first DC are tested.
+// FP32 rows of up to 12 dimensions are read in one batch (one wait).
 template <int DC, int D, typename T>
 __device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ tile, int lane,
                                                         const WaveBox<DC, T>& box) {
+  constexpr int B = sizeof(T) == 4 && DC <= 12 ? DC : kTestBatch;
   const T* row = tile + lane * D * 2;
   unsigned h = 1u;
 #pragma unroll
-  for (int d0 = 0; d0 < DC; d0 += kTestBatch) {
-    T r[2 * kTestBatch];
+  for (int d0 = 0; d0 < DC; d0 += B) {
+    T r[2 * B];
 #pragma unroll
-    for (int e = 0; e < 2 * kTestBatch; ++e)
+    for (int e = 0; e < 2 * B; ++e)
       if (2 * d0 + e < 2 * DC) r[e] = row[2 * d0 + e];
 #pragma unroll
-    for (int dd = 0; dd < kTestBatch; ++dd)
+    for (int dd = 0; dd < B; ++dd)
       if (d0 + dd < DC)
         h &= static_cast<unsigned>(may_hit(r[2 * dd], r[2 * dd + 1], box.l[d0 + dd], box.h[d0 + dd]));
   }

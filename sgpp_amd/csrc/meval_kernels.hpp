@@ -105,12 +105,20 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
   const int jb = blockIdx.y * chunk;
   const int je = min(jb + chunk, jend_all);
   unsigned long long n_steps = 0, n_pairs = 0;
-  for (int j0 = jb; j0 < je; j0 += kTG) {
+  // windows of up to 64 tiles: one lane per tile tests the tile's box
+  for (int w0 = jb; w0 < je; w0 += 64 * kTG) {
+  unsigned long long tmask = tiles_may_hit<D, D, T>(
+      TB, w0 / kTG, min(64, (je - w0 + kTG - 1) / kTG), lane, box);
+  while (tmask) {
+    const int j0 = w0 + __builtin_ctzll(tmask) * kTG;
+    tmask &= tmask - 1;
     const int64_t t = j0 / kTG;
-    if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
     const T* tile = LI + t * D * kTG * 2;
+    // this lane's point's dimensions, loaded with the tile test's rows (the
+    // empty asm uses it there, so the load is not sunk to its first use)
+    const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;
     unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
-    const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;  // this lane's point's dimensions
+    if (kSkip) asm volatile("" ::"v"(pm));
     T acc[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] = T(0);
@@ -133,6 +141,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
+  }
   }
   double* out = part + static_cast<int64_t>(blockIdx.y) * ldx;
 #pragma unroll
@@ -212,12 +221,20 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
     }
     WaveBox<D, T> box;
     wave_box<D, D, K, T>(x, box);
-    for (int j0 = jb; j0 < je; j0 += kTG) {
+    // the chunk's tiles (kChunkT / kTG <= 64), one lane per tile
+    static_assert(kChunkT / kTG <= 64, "one window of tiles per chunk");
+    unsigned long long tmask =
+        tiles_may_hit<D, D, T>(TB, jb / kTG, (je - jb + kTG - 1) / kTG, lane, box);
+    while (tmask) {
+      const int j0 = jb + __builtin_ctzll(tmask) * kTG;
+      tmask &= tmask - 1;
       const int64_t t = j0 / kTG;
-      if (!tile_may_hit<D, D, T>(TB, t, box)) continue;
       const T* tile = LI + t * D * kTG * 2;
+      // this lane's point's dimensions, loaded with the tile test's rows (the
+      // empty asm uses it there, so the load is not sunk to its first use)
+      const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;
       unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
-      const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;  // this lane's point's dimensions
+      if (kSkip) asm volatile("" ::"v"(pm));
       // a grid point is taken at most once per (group, tile): its sum goes
       // to lane jj of tsum, and the tile is added to the wave's row once
       T tsum = T(0);
