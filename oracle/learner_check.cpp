@@ -30,7 +30,7 @@
 // branch a maintainer adds there to make HIP the default.
 //
 //   learner_check <learner|fitter|regression|weighted>
-//                 <streaming|default|hip64|hip32|hipdefault> <dir> key=value...
+//                 <streaming|default|subspace|hip64|hip32|hipsubspace|hipdefault> <dir> key=value...
 // dir holds X.bin (M x d), y.bin (M), Xt.bin (Mt x d), raw little-endian
 // float64. Keys: basis (0 Linear, 1 ModLinear), dim, level, M, Mt, lambda,
 // eps_refine, it_refine, eps_final, it_final, refinements, points, threshold,
@@ -169,10 +169,15 @@ OperationMultipleEvalConfiguration config_for(const std::string& op) {
   if (op == "streaming")
     return OperationMultipleEvalConfiguration(OperationMultipleEvalType::STREAMING,
                                               OperationMultipleEvalSubType::DEFAULT);
-  if (op == "hip64" || op == "hip32") {
+  if (op == "subspace")  // the reference's OperationMultipleEvalSubspaceCombined
+    return OperationMultipleEvalConfiguration(OperationMultipleEvalType::SUBSPACELINEAR,
+                                              OperationMultipleEvalSubType::COMBINED);
+  if (op == "hip64" || op == "hip32" || op == "hipsubspace") {
     sgpp::base::OperationConfiguration params;
     params.addTextAttr("INTERNAL_PRECISION", op == "hip32" ? "float" : "double");
-    return OperationMultipleEvalConfiguration(OperationMultipleEvalType::STREAMING,
+    return OperationMultipleEvalConfiguration(op == "hipsubspace"
+                                                  ? OperationMultipleEvalType::SUBSPACELINEAR
+                                                  : OperationMultipleEvalType::STREAMING,
                                               sgpp::datadriven::kOperationMultipleEvalSubTypeHIP,
                                               params, "HIP");
   }

@@ -54,6 +54,25 @@ def test_reference_learner_on_the_hip_operator(gpu, basis):
     assert hinfo["train_mse"] == pytest.approx(rinfo["train_mse"], rel=1e-8)
 
 
+def test_reference_learner_on_the_hip_subspace_operator(gpu):
+    """The factory's SUBSPACELINEAR branch: the reference's learner on the
+    adapter's subspace-indexed kernels against the same learner on the
+    reference's own OperationMultipleEvalSubspaceCombined (Linear grids)."""
+    X, y = dataset(55)
+    Xt, _ = dataset(56, M=400)
+    cfg = dict(refine=(1e-12, 1000), num_refinements=3, num_points=5)
+    ref, rinfo = oracle.run_learner_check("learner", "subspace", 0, X, y, 3, 1e-4, Xt=Xt, **cfg)
+    hip, hinfo = oracle.run_learner_check("learner", "hipsubspace", 0, X, y, 3, 1e-4, Xt=Xt,
+                                          **cfg)
+    assert hinfo["hip_ops"] >= 2 and rinfo["hip_ops"] == 0
+    rlv, rix, ra, rpred = ref["train"]
+    lv, ix, a, pred = hip["train"]
+    assert np.array_equal(lv, rlv) and np.array_equal(ix, rix)
+    assert rel(a, ra) <= 1e-8
+    assert rel(pred, rpred) <= 1e-8
+    assert hinfo["train_mse"] == pytest.approx(rinfo["train_mse"], rel=1e-8)
+
+
 def test_reference_learner_fp32_products(gpu):
     X, y = dataset(53, d=4)
     Xt, _ = dataset(54, M=400, d=4)
