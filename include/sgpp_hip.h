@@ -258,7 +258,18 @@ typedef struct sgpp_hip_server_config {
   double item_work;             /* untuned: lane-operations per work item; 0 = 2^25 */
   int autotune;                 /* time tile shapes per (basis, dimension) at admission */
   const char* autotune_cache;   /* file of tuned shapes (read, then updated); NULL = none */
+  int algorithm;                /* SGPP_HIP_SERVE_*; 0 = per model the faster */
 } sgpp_hip_server_config;
+
+/* How a served model is evaluated. STREAMING: every (query, grid point)
+ * pair, in the configured precision. SUBSPACE: per query, each subspace of
+ * the model once, the one basis function whose support holds the query taken
+ * from a dense index table, in FP64 (the SubspaceLinear algorithm,
+ * OperationMultipleEvalSubspaceCombined.hpp:92-204); models whose tables
+ * would be too large stay on STREAMING. AUTO: with autotune, the faster of
+ * the two, timed per model at admission; without it, STREAMING. Callers
+ * compiled against a header without the `algorithm` member must be rebuilt. */
+enum { SGPP_HIP_SERVE_AUTO = 0, SGPP_HIP_SERVE_STREAMING = 1, SGPP_HIP_SERVE_SUBSPACE = 2 };
 
 /* A tuned shape: grid points per work item and the measured seconds per
  * (query x grid point). */
@@ -319,6 +330,10 @@ sgpp_hip_status sgpp_hip_server_stats(sgpp_hip_server* s, sgpp_hip_server_statis
  * application/StaticParameterTuner.cpp:234-367). */
 sgpp_hip_status sgpp_hip_server_tuned_shape(sgpp_hip_server* s, int basis, int dim, int* found,
                                             sgpp_hip_server_shape* out);
+/* *algorithm = SGPP_HIP_SERVE_STREAMING or SGPP_HIP_SERVE_SUBSPACE: the kernel
+ * that serves the model (no reference counterpart) */
+sgpp_hip_status sgpp_hip_server_model_algorithm(sgpp_hip_server* s, int model_id,
+                                                int* algorithm);
 
 /* --------------------------------------------------------- learner
  * The regression learner that drives the path: regular grid -> (refine ->

@@ -23,6 +23,7 @@ STATUS_NAMES = {0: "OK", 1: "INVALID_ARGUMENT", 2: "FACTORY", 3: "OPERATION", 4:
 BASIS_LINEAR, BASIS_MODLINEAR = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 ALGO_STREAMING, ALGO_SUBSPACE = 0, 1
+SERVE_AUTO, SERVE_STREAMING, SERVE_SUBSPACE = 0, 1, 2
 SOLVE_FP64, SOLVE_MIXED, SOLVE_FP32 = 0, 1, 2
 SLE_CG, SLE_BICGSTAB = 0, 1
 
@@ -187,6 +188,7 @@ SIGNATURES = {
     "sgpp_hip_server_stats": (_st, [_p, ctypes.c_void_p]),
     "sgpp_hip_server_tuned_shape": (_st, [_p, ctypes.c_int, ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
+    "sgpp_hip_server_model_algorithm": (_st, [_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "sgpp_hip_learner_create": (_st, [ctypes.c_int, ctypes.POINTER(Params), ctypes.POINTER(_p)]),
     "sgpp_hip_learner_destroy": (_st, [_p]),
     "sgpp_hip_learner_train": (_st, [_p, _dp, _sz, _sz, _dp, ctypes.POINTER(GridConfig),
@@ -492,7 +494,7 @@ class ServerConfig(ctypes.Structure):
                 ("max_inflight", ctypes.c_int), ("max_requests_per_batch", ctypes.c_int),
                 ("blocks", ctypes.c_int), ("use_graph", ctypes.c_int),
                 ("item_work", ctypes.c_double), ("autotune", ctypes.c_int),
-                ("autotune_cache", ctypes.c_char_p)]
+                ("autotune_cache", ctypes.c_char_p), ("algorithm", ctypes.c_int)]
 
 
 class ServerShape(ctypes.Structure):
@@ -506,15 +508,20 @@ class ServerStatistics(ctypes.Structure):
 
 
 class Server:
-    """Resident models + continuous batching (sgpp_hip_server_*)."""
+    """Resident models + continuous batching (sgpp_hip_server_*).
+
+    algorithm: SERVE_AUTO (per model the faster of the streaming and the
+    subspace kernel when autotune is on, else streaming), SERVE_STREAMING or
+    SERVE_SUBSPACE (wherever the model has a subspace layout)."""
 
     def __init__(self, precision=PREC_FP32, memory_budget=0, max_batch_queries=0, max_items=0,
                  max_partial_doubles=0, max_inflight=2, max_requests_per_batch=0, blocks=0,
-                 use_graph=True, item_work=0.0, device=0, autotune=False, autotune_cache=None):
+                 use_graph=True, item_work=0.0, device=0, autotune=False, autotune_cache=None,
+                 algorithm=0):
         self._cache = autotune_cache.encode() if autotune_cache else None
         cfg = ServerConfig(device, precision, memory_budget, max_batch_queries, max_items,
                            max_partial_doubles, max_inflight, max_requests_per_batch, blocks,
-                           int(use_graph), item_work, int(autotune), self._cache)
+                           int(use_graph), item_work, int(autotune), self._cache, algorithm)
         h = _p()
         check(F["sgpp_hip_server_create"](ctypes.cast(ctypes.byref(cfg), ctypes.c_void_p),
                                           ctypes.byref(h)))
@@ -597,6 +604,12 @@ class Server:
         s = ServerStatistics()
         check(F["sgpp_hip_server_stats"](self.h, ctypes.cast(ctypes.byref(s), ctypes.c_void_p)))
         return {k: getattr(s, k) for k, _ in ServerStatistics._fields_}
+
+    def model_algorithm(self, model_id):
+        """SERVE_STREAMING or SERVE_SUBSPACE: the kernel that serves the model."""
+        a = ctypes.c_int(0)
+        check(F["sgpp_hip_server_model_algorithm"](self.h, model_id, ctypes.byref(a)))
+        return a.value
 
     def tuned_shape(self, basis, dim):
         """{"chunk", "cost"} of the tuned shape of (basis, dim), or None."""

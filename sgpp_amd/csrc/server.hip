@@ -13,6 +13,7 @@
 
 #include "cull_device.hpp"
 #include "server.hpp"
+#include "subspace_device.hpp"
 
 namespace sgpp_hip {
 
@@ -31,6 +32,11 @@ struct ModelDesc {
   const uint32_t* hmask;  // [Npad/J] dimensions in which a sub-tile point has L >= kLoCorrectL
   int D;
   int Npad;               // multiple of kTG
+  // subspace algorithm (null / 0 when the model has no subspace layout)
+  const SubspaceDesc* SR;  // records sorted by level vector (+ kSubBatch - 1 padding)
+  const double* TW;        // per table entry: alpha of its grid point, 0 where none
+  int nsub;                // subspaces
+  int mod;                 // ModLinear basis
 };
 
 // FP32 serving: a query coordinate x (FP64) is carried as hi = fl32(x) and
@@ -65,6 +71,14 @@ struct Span {
   long long out0;  // output offset of the span's first query
 };
 static_assert(sizeof(Span) == 64, "one span is 64 bytes");
+
+// Subspace-algorithm spans: the same layout, with `chunk` subspaces per
+// item, `nchunks` = ceil(nsub / chunk) and one query per lane (tile = kBlock);
+// their items are numbered apart from the streaming items (counts below).
+// counts[0] = streaming items, [1] = reduction slots, [2] = streaming spans
+// (spans [0, counts[2])), [3] = subspace items, [4] = all spans (the subspace
+// spans are [counts[2], counts[4])).
+constexpr int kCounts = 8;
 
 // one work item, decoded from its span
 struct Item {
@@ -364,10 +378,92 @@ void serve_kernel(const ModelDesc* __restrict__ models, const Span* __restrict__
   }
 }
 
+// One subspace item: queries [q0, q0 + kBlock) of a span (one per lane)
+// against subspaces [sb, se) of its model, in FP64: x = hi (+ lo for FP32
+// queries), per subspace the changed dimensions, the weight gathered from the
+// table. Records are loaded kSubBatch at a time; the first subspace of the
+// item starts from scratch. Padding queries (zero coordinates) give values
+// the reduction ignores.
+template <int D, bool kMod, typename T>
+__device__ void serve_sub_item(const ModelDesc& md, const T* __restrict__ X, long long xoff,
+                               int ldq, int lo, int q0, int sb, int se, double* __restrict__ out) {
+  const long long q = q0 + static_cast<int>(threadIdx.x);
+  double x[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    x[d] = static_cast<double>(X[xoff + static_cast<long long>(d) * ldq + q]);
+    if (sizeof(T) == 4 && lo)
+      x[d] += static_cast<double>(X[xoff + static_cast<long long>(D + d) * ldq + q]);
+  }
+  double prod[D + 1];
+  unsigned flat[D + 1];
+  prod[0] = 1.0;
+  flat[0] = 0u;
+  double acc = 0.0;
+  for (int s = sb; s < se; ++s) {
+    const SubspaceDesc sd = md.SR[s];
+    walk_subspace<D, kMod>(x, sd, s == sb ? 0 : sd.recalc, prod, flat);
+    acc = __builtin_fma(md.TW[sd.off + flat[D]], prod[D], acc);
+  }
+  out[threadIdx.x] = acc;
+}
+
+template <int D, typename T>
+__device__ __forceinline__ void dispatch_sub(const ModelDesc& md, const T* X, const Span& sp,
+                                             int q0, int sb, int se, double* out) {
+  if (md.mod)
+    serve_sub_item<D, true, T>(md, X, sp.xoff, sp.ldq, sp.lo, q0, sb, se, out);
+  else
+    serve_sub_item<D, false, T>(md, X, sp.xoff, sp.ldq, sp.lo, q0, sb, se, out);
+}
+
+// Persistent work-list kernel of the subspace items (counts[3] items of the
+// spans [counts[2], counts[4])), pulling from counter[1].
+template <typename T>
+__global__ __launch_bounds__(kBlock) void serve_sub_kernel(const ModelDesc* __restrict__ models,
+                                                           const Span* __restrict__ spans,
+                                                           const int* __restrict__ counts,
+                                                           unsigned* counter,
+                                                           const T* __restrict__ X,
+                                                           double* __restrict__ part) {
+  __shared__ unsigned s_item;
+  const int n = counts[3];
+  const Span* sspans = spans + counts[2];
+  const int nspans = counts[4] - counts[2];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = atomicAdd(counter + 1, 1u);
+    __syncthreads();
+    const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(s_item));
+    __syncthreads();
+    if (it >= n || it < 0) break;
+    const Span sp = sspans[find_span(nspans, it, [&](int k) { return sspans[k].item0; })];
+    const ModelDesc md = models[sp.model];
+    const int local = it - sp.item0;
+    const int t = local / sp.nchunks;
+    const int c = local - t * sp.nchunks;
+    const int sb = c * sp.chunk;
+    const int se = min(sb + sp.chunk, md.nsub);
+    double* out = part + sp.pbase + static_cast<long long>(local) * sp.tile;
+    switch (sp.D) {
+#define SGPP_CASE(DD) \
+  case DD:            \
+    dispatch_sub<DD, T>(md, X, sp, t * sp.tile, sb, se, out); \
+    break;
+      SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
+      SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
+      SGPP_CASE(14) SGPP_CASE(15) SGPP_CASE(16) SGPP_CASE(17) SGPP_CASE(18) SGPP_CASE(19)
+      SGPP_CASE(20)
+#undef SGPP_CASE
+      default:
+        break;
+    }
+  }
+}
+
 // out[query] = the sum of the query's partials over its model's grid chunks,
 // in chunk order. Each block takes kBlock consecutive reduction slots (all in
 // one span); each thread keeps kRedBatch partial loads in flight and adds
-// them in order. counts[1] = slots, counts[2] = spans.
+// them in order. counts[1] = slots, counts[4] = spans.
 constexpr int kRedBatch = 8;
 constexpr int kReduceBlocksPerCU = 8;
 __global__ __launch_bounds__(kBlock) void serve_reduce(const Span* __restrict__ spans,
@@ -375,7 +471,7 @@ __global__ __launch_bounds__(kBlock) void serve_reduce(const Span* __restrict__ 
                                                        const double* __restrict__ part,
                                                        double* __restrict__ out) {
   const int nslots = counts[1];
-  const int nspans = counts[2];
+  const int nspans = counts[4];
   for (int s0 = blockIdx.x * kBlock; s0 < nslots; s0 += gridDim.x * kBlock) {
     const Span sp = spans[find_span(nspans, s0, [&](int k) { return spans[k].slot0; })];
     const int local = s0 - sp.slot0 + static_cast<int>(threadIdx.x);
@@ -398,8 +494,9 @@ __global__ __launch_bounds__(kBlock) void serve_reduce(const Span* __restrict__ 
   }
 }
 
+// both work-list counters (streaming items, subspace items)
 __global__ void reset_counter(unsigned* counter) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+  if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
 }
 
 // Span of q queries of one model (stored [D][ldq] at xoff, ldq a multiple of
@@ -456,15 +553,22 @@ struct Server::Model {
   void* pm = nullptr;  // [Npad/4] dimensions evaluated per 4 consecutive points
   void* um = nullptr;  // [Npad/J] dimensions whose (L, I) the J points of a sub-tile share
   void* hm = nullptr;  // [Npad/J] dimensions of a sub-tile that add the queries' low parts
-  bool lo = false;     // FP32 server and some hm bit set: queries carry low parts
+  bool lo = false;     // FP32 server and (some hm bit set or sub): queries carry low parts
+  // subspace algorithm: records and table weights (when the layout exists),
+  // whether the model is served by it, and subspaces per work item
+  void* sr = nullptr;
+  void* tw = nullptr;
+  int nsub = 0;
+  bool sub = false;
+  int sub_chunk = 0;
   size_t bytes = 0;
   int active = 0;    // requests queued or running
   int inflight = 0;  // pieces of its requests in formed or running batches
   void free_device() {
-    void* p[] = {LI, A, pm, um, hm};
+    void* p[] = {LI, A, pm, um, hm, sr, tw};
     for (void* q : p)
       if (q) (void)hipFree(q);
-    LI = A = pm = um = hm = nullptr;
+    LI = A = pm = um = hm = sr = tw = nullptr;
   }
 };
 
@@ -517,10 +621,40 @@ struct Server::Slot {
   };
   std::vector<Block> blocks;  // one per model in the batch (padding columns)
   size_t usedX = 0, nItems = 0, nSlots = 0, nSpans = 0, usedPart = 0, usedOut = 0;
+  size_t nSubItems = 0, nStreamSpans = 0;  // subspace items; spans of the streaming kernel
 };
+
+// reset, streaming work-list kernel, subspace work-list kernel, reduction of
+// slot s's batch on stream st (also what the slot's graph holds)
+void Server::enqueue_batch_kernels(Slot& s, hipStream_t st) {
+  enqueue_kernels(st, s.dSpans, s.dCounts, s.dCounter, s.dX, s.dPart, s.dOut);
+}
+
+void Server::enqueue_kernels(hipStream_t st, const void* spans, const int* counts,
+                             unsigned* counter, const void* X, double* part, double* out) {
+  const ModelDesc* models = static_cast<const ModelDesc*>(model_table_);
+  const Span* sp = static_cast<const Span*>(spans);
+  clear_launch_error();
+  hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, st, counter);
+  if (cfg_.precision == Precision::FP64) {
+    hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models, sp,
+                       counts, counter, static_cast<const double*>(X), part);
+    hipLaunchKernelGGL(serve_sub_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models,
+                       sp, counts, counter, static_cast<const double*>(X), part);
+  } else {
+    hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models, sp,
+                       counts, counter, static_cast<const float*>(X), part);
+    hipLaunchKernelGGL(serve_sub_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models,
+                       sp, counts, counter, static_cast<const float*>(X), part);
+  }
+  hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0, st, sp,
+                     counts, part, out);
+}
 
 Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
   if (cfg_.max_inflight < 1 || cfg_.max_inflight > 8) throw input_error("max_inflight in [1, 8]");
+  if (cfg_.algorithm < kServeAuto || cfg_.algorithm > kServeSubspace)
+    throw input_error("unknown serving algorithm");
   // item and slot numbers are ints on the device
   if (cfg_.max_items > (size_t(1) << 30) || cfg_.max_batch_queries > (size_t(1) << 28))
     throw input_error("max_items <= 2^30 and max_batch_queries <= 2^28");
@@ -539,7 +673,7 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     s->capItems = cfg_.max_items;
     s->capSpans = kMaxModels;  // one span per model and batch
     s->capPart = cfg_.max_partial_doubles;
-    const size_t dev_bytes = s->capX * esz + s->capSpans * sizeof(Span) + 4 * sizeof(int) +
+    const size_t dev_bytes = s->capX * esz + s->capSpans * sizeof(Span) + kCounts * sizeof(int) +
                              s->capPart * sizeof(double) + s->capOut * sizeof(double);
     if (used_bytes_ + dev_bytes > cfg_.memory_budget)
       throw input_error("memory budget too small for the batch buffers");
@@ -550,9 +684,9 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     check_hip(hipMalloc(&s->dX, s->capX * esz), "hipMalloc(batch X)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dSpans), s->capSpans * sizeof(Span)),
               "hipMalloc(spans)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounts), 4 * sizeof(int)),
+    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounts), kCounts * sizeof(int)),
               "hipMalloc(counts)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounter), sizeof(unsigned)),
+    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounter), 2 * sizeof(unsigned)),
               "hipMalloc(counter)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dPart), s->capPart * sizeof(double)),
               "hipMalloc(partials)");
@@ -561,28 +695,18 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hX), s->capX * esz), "hipHostMalloc");
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hSpans), s->capSpans * sizeof(Span)),
               "hipHostMalloc");
-    check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hCounts), 4 * sizeof(int)),
+    check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hCounts), kCounts * sizeof(int)),
               "hipHostMalloc");
     check_hip(hipHostMalloc(reinterpret_cast<void**>(&s->hOut), s->capOut * sizeof(double)),
               "hipHostMalloc");
     if (cfg_.use_graph) {
-      // reset + persistent work-list kernel + reduction, captured once per slot;
-      // the item counts are read from device memory, so one graph serves every batch
+      // reset + the two persistent work-list kernels + reduction, captured once
+      // per slot; the item counts are read from device memory, so one graph
+      // serves every batch
       hipGraph_t g;
       check_hip(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal),
                 "hipStreamBeginCapture");
-      clear_launch_error();
-      hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s->stream, s->dCounter);
-      if (cfg_.precision == Precision::FP64)
-        hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s->stream,
-                           static_cast<const ModelDesc*>(model_table_), s->dSpans, s->dCounts,
-                           s->dCounter, static_cast<const double*>(s->dX), s->dPart);
-      else
-        hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, s->stream,
-                           static_cast<const ModelDesc*>(model_table_), s->dSpans, s->dCounts,
-                           s->dCounter, static_cast<const float*>(s->dX), s->dPart);
-      hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
-                         s->stream, s->dSpans, s->dCounts, s->dPart, s->dOut);
+      enqueue_batch_kernels(*s, s->stream);
       check_hip(hipStreamEndCapture(s->stream, &g), "hipStreamEndCapture");
       check_hip(hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
       check_hip(hipGraphDestroy(g), "hipGraphDestroy");
@@ -712,6 +836,24 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       hmk[t] = h;
       if (h) m->lo = true;
     }
+  // subspace layout (records and FP64 table weights in the caller's point
+  // order) unless the server runs the streaming kernel only; a grid whose
+  // tables would be too large has none
+  SubspaceLayout lay;
+  std::vector<double> tw;
+  if (cfg_.algorithm != kServeStreaming) {
+    try {
+      lay = build_subspace_layout(dim, N, level, index);
+      tw.assign(lay.ti.size(), 0.0);
+      for (size_t e = 0; e < tw.size(); ++e)
+        if (lay.ti[e] >= 0) tw[e] = alpha[lay.ti[e]];
+      m->nsub = lay.nsub;
+      m->bytes += lay.desc.size() * sizeof(SubspaceDesc) + tw.size() * sizeof(double);
+    } catch (const input_error&) {
+      lay = SubspaceLayout();
+      tw.clear();
+    }
+  }
   {
     std::lock_guard<std::mutex> lk(mu_);
     if (used_bytes_ + m->bytes > cfg_.memory_budget)
@@ -754,9 +896,27 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
       check_hip(hipMemcpy(m->LI, l32.data(), l32.size() * 4, hipMemcpyHostToDevice), "upload LI");
       check_hip(hipMemcpy(m->A, a32.data(), a32.size() * 4, hipMemcpyHostToDevice), "upload A");
     }
-    ModelDesc md{m->LI, m->A, static_cast<const uint32_t*>(m->pm),
-                 static_cast<const uint32_t*>(m->um), static_cast<const uint32_t*>(m->hm), m->D,
-                 static_cast<int>(m->Npad)};
+    if (m->nsub > 0) {
+      check_hip(hipMalloc(&m->sr, lay.desc.size() * sizeof(SubspaceDesc)),
+                "hipMalloc(model subspaces)");
+      check_hip(hipMemcpy(m->sr, lay.desc.data(), lay.desc.size() * sizeof(SubspaceDesc),
+                          hipMemcpyHostToDevice),
+                "upload subspaces");
+      check_hip(hipMalloc(&m->tw, tw.size() * sizeof(double)), "hipMalloc(model tables)");
+      check_hip(hipMemcpy(m->tw, tw.data(), tw.size() * sizeof(double), hipMemcpyHostToDevice),
+                "upload tables");
+    }
+    ModelDesc md{m->LI,
+                 m->A,
+                 static_cast<const uint32_t*>(m->pm),
+                 static_cast<const uint32_t*>(m->um),
+                 static_cast<const uint32_t*>(m->hm),
+                 m->D,
+                 static_cast<int>(m->Npad),
+                 static_cast<const SubspaceDesc*>(m->sr),
+                 static_cast<const double*>(m->tw),
+                 m->nsub,
+                 basis == Basis::ModLinear ? 1 : 0};
     check_hip(hipMemcpy(static_cast<ModelDesc*>(model_table_) + m->table_index, &md,
                         sizeof(ModelDesc), hipMemcpyHostToDevice),
               "upload model descriptor");
@@ -767,27 +927,25 @@ int Server::admit_model(Basis basis, size_t dim, size_t N, const uint32_t* level
     m->free_device();
     throw;
   }
-  if (cfg_.autotune) {
-    bool have;
+  {
+    bool have = true;
     {
       std::lock_guard<std::mutex> lk(mu_);
-      have = tuned_.count(tune_key(basis, m->D)) > 0;
+      if (cfg_.autotune) have = tuned_.count(tune_key(basis, m->D)) > 0;
+      m->active++;  // not evictable while its launches run
     }
-    if (!have) {
-      {
-        std::lock_guard<std::mutex> lk(mu_);
-        m->active++;  // not evictable while its launches run
-      }
-      try {
-        tune(*m);
-      } catch (...) {
-        std::lock_guard<std::mutex> lk(mu_);
-        m->active--;
-        throw;
-      }
+    try {
+      if (!have) tune(*m);
+      choose_algorithm(*m);
+    } catch (...) {
       std::lock_guard<std::mutex> lk(mu_);
       m->active--;
+      throw;
     }
+    std::lock_guard<std::mutex> lk(mu_);
+    // the subspace kernel reads FP64 queries: an FP32 server carries their low parts
+    if (m->sub && !f64) m->lo = true;
+    m->active--;
   }
   return m->id;
 }
@@ -947,6 +1105,7 @@ void Server::fail_pieces(Slot& s, const char* why) {
 // does is placed and the request stays at the front for the next batch.
 bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
   s.usedX = s.nItems = s.nSlots = s.nSpans = s.usedPart = s.usedOut = 0;
+  s.nSubItems = s.nStreamSpans = 0;
   s.blocks.clear();
   const double t = now_s();
   struct Group {
@@ -976,9 +1135,16 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
       continue;
     }
     const Model& m = *models_.at(r->model);
-    Group probe{&m, static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D), chunk_for(m), 0,
-                0, {}};
-    probe.nchunks = (m.Npad + probe.chunk - 1) / probe.chunk;
+    // subspace models: one query per lane, sub_chunk subspaces per item
+    Group probe{&m,
+                m.sub ? static_cast<size_t>(kBlock)
+                      : static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D),
+                m.sub ? static_cast<size_t>(m.sub_chunk) : chunk_for(m),
+                0,
+                0,
+                {}};
+    probe.nchunks = m.sub ? (static_cast<size_t>(m.nsub) + probe.chunk - 1) / probe.chunk
+                          : (m.Npad + probe.chunk - 1) / probe.chunk;
     size_t x1, i1, p1;
     need(probe, 1, &x1, &i1, &p1);
     if (i1 > s.capItems || p1 > s.capPart || x1 > s.capX) {
@@ -1033,25 +1199,33 @@ bool Server::form_batch(Slot& s, std::vector<Piece>& taken) {
     }
     if (n < rest) break;  // the batch is full
   }
-  for (Group& g : groups) {
-    const size_t ntiles = (g.q + g.tile - 1) / g.tile;
-    const size_t ldq = ntiles * g.tile;
-    const long long xoff = static_cast<long long>(s.usedX);
-    const long long out0 = static_cast<long long>(s.usedOut);
-    for (size_t k : g.pieces) {
-      Piece& pc = taken[k];
-      pc.xoff = xoff;
-      pc.ldq = ldq;
-      pc.out_off = out0 + static_cast<long long>(pc.qoff);
+  // the streaming kernel's spans first, then the subspace kernel's; each
+  // kind numbers its items from 0, slots and partials run on across both
+  for (int pass = 0; pass < 2; ++pass) {
+    for (Group& g : groups) {
+      if (g.m->sub != (pass == 1)) continue;
+      const size_t ntiles = (g.q + g.tile - 1) / g.tile;
+      const size_t ldq = ntiles * g.tile;
+      const long long xoff = static_cast<long long>(s.usedX);
+      const long long out0 = static_cast<long long>(s.usedOut);
+      for (size_t k : g.pieces) {
+        Piece& pc = taken[k];
+        pc.xoff = xoff;
+        pc.ldq = ldq;
+        pc.out_off = out0 + static_cast<long long>(pc.qoff);
+      }
+      size_t& item0 = g.m->sub ? s.nSubItems : s.nItems;
+      s.hSpans[s.nSpans++] = make_span(
+          g.m->table_index, g.m->D, g.m->sub ? static_cast<size_t>(g.m->nsub) : g.m->Npad, g.tile,
+          g.chunk, g.q, ldq, xoff, item0, s.nSlots, s.usedPart, out0, g.m->lo);
+      s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq, g.m->lo});
+      item0 += ntiles * g.nchunks;
+      s.nSlots += ldq;
+      s.usedPart += ntiles * g.nchunks * g.tile;
+      s.usedX += static_cast<size_t>(g.m->D) * ldq * (g.m->lo ? 2 : 1);
+      s.usedOut += g.q;
     }
-    s.hSpans[s.nSpans++] = make_span(g.m->table_index, g.m->D, g.m->Npad, g.tile, g.chunk, g.q,
-                                     ldq, xoff, s.nItems, s.nSlots, s.usedPart, out0, g.m->lo);
-    s.blocks.push_back(Slot::Block{xoff, g.m->D, g.q, ldq, g.m->lo});
-    s.nItems += ntiles * g.nchunks;
-    s.nSlots += ldq;
-    s.usedPart += ntiles * g.nchunks * g.tile;
-    s.usedX += static_cast<size_t>(g.m->D) * ldq * (g.m->lo ? 2 : 1);
-    s.usedOut += g.q;
+    if (pass == 0) s.nStreamSpans = s.nSpans;
   }
   return !taken.empty();
 }
@@ -1072,31 +1246,22 @@ void Server::launch_batch(Slot& s) {
       std::memset(s.hX + (b.xoff + d * b.ldq + b.q) * esz, 0, (b.ldq - b.q) * esz);
   s.hCounts[0] = static_cast<int>(s.nItems);
   s.hCounts[1] = static_cast<int>(s.nSlots);
-  s.hCounts[2] = static_cast<int>(s.nSpans);
-  s.hCounts[3] = 0;
+  s.hCounts[2] = static_cast<int>(s.nStreamSpans);
+  s.hCounts[3] = static_cast<int>(s.nSubItems);
+  s.hCounts[4] = static_cast<int>(s.nSpans);
+  for (int k = 5; k < kCounts; ++k) s.hCounts[k] = 0;
   check_hip(hipEventRecord(s.ev_start, s.stream), "hipEventRecord");
   check_hip(hipMemcpyAsync(s.dX, s.hX, s.usedX * esz, hipMemcpyHostToDevice, s.stream), "H2D X");
   check_hip(hipMemcpyAsync(s.dSpans, s.hSpans, s.nSpans * sizeof(Span), hipMemcpyHostToDevice,
                            s.stream),
             "H2D spans");
-  check_hip(hipMemcpyAsync(s.dCounts, s.hCounts, 4 * sizeof(int), hipMemcpyHostToDevice,
+  check_hip(hipMemcpyAsync(s.dCounts, s.hCounts, kCounts * sizeof(int), hipMemcpyHostToDevice,
                            s.stream),
             "H2D counts");
   if (s.graph) {
     check_hip(hipGraphLaunch(s.graph, s.stream), "hipGraphLaunch");
   } else {
-    clear_launch_error();
-    hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, s.stream, s.dCounter);
-    if (cfg_.precision == Precision::FP64)
-      hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, s.stream,
-                         static_cast<const ModelDesc*>(model_table_), s.dSpans, s.dCounts,
-                         s.dCounter, static_cast<const double*>(s.dX), s.dPart);
-    else
-      hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, s.stream,
-                         static_cast<const ModelDesc*>(model_table_), s.dSpans, s.dCounts,
-                         s.dCounter, static_cast<const float*>(s.dX), s.dPart);
-    hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
-                       s.stream, s.dSpans, s.dCounts, s.dPart, s.dOut);
+    enqueue_batch_kernels(s, s.stream);
     check_hip(hipGetLastError(), "serve launch");
   }
   check_hip(hipMemcpyAsync(s.hOut, s.dOut, s.usedOut * sizeof(double), hipMemcpyDeviceToHost,
@@ -1133,7 +1298,7 @@ void Server::finish_batch(Slot& s) {
     stats_.queries_done += r.q;
   }
   stats_.batches++;
-  stats_.kernel_launches += cfg_.use_graph ? 1 : 3;
+  stats_.kernel_launches += cfg_.use_graph ? 1 : 4;
   s.pieces.clear();
   s.busy = false;
   s.inject = false;
@@ -1155,6 +1320,13 @@ size_t Server::chunk_for(const Model& m) const {
     chunk = static_cast<size_t>(std::ceil(cfg_.item_work / per_point / kTG)) * kTG;
   }
   return std::max<size_t>(kTG, std::min(chunk, m.Npad));
+}
+
+int Server::model_algorithm(int id) {
+  std::lock_guard<std::mutex> lk(mu_);
+  auto it = models_.find(id);
+  if (it == models_.end()) throw input_error("unknown model");
+  return it->second->sub ? kServeSubspace : kServeStreaming;
 }
 
 bool Server::tuned_shape(Basis basis, int dim, TunedShape* out) {
@@ -1228,8 +1400,8 @@ void Server::tune(Model& m) {
     check_hip(hipMalloc(reinterpret_cast<void**>(&dSpans), sizeof(Span)), "hipMalloc(tune span)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dOut), nq * sizeof(double)),
               "hipMalloc(tune out)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), 4 * sizeof(int)), "hipMalloc");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), sizeof(unsigned)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), kCounts * sizeof(int)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), 2 * sizeof(unsigned)), "hipMalloc");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dPart),
                         std::max(kTunePartDoubles, ntiles * tile) * sizeof(double)),
               "hipMalloc(tune partials)");
@@ -1240,25 +1412,14 @@ void Server::tune(Model& m) {
       const size_t nitems = ntiles * nchunks;
       const Span sp = make_span(m.table_index, m.D, m.Npad, tile, static_cast<size_t>(c), nq, nq,
                                 0, 0, 0, 0, 0, m.lo);
-      const int counts[4] = {static_cast<int>(nitems), static_cast<int>(nq), 1, 0};
+      const int counts[kCounts] = {static_cast<int>(nitems), static_cast<int>(nq), 1, 0, 1};
       check_hip(hipMemcpy(dSpans, &sp, sizeof(Span), hipMemcpyHostToDevice), "upload tune span");
       check_hip(hipMemcpy(dCounts, counts, sizeof(counts), hipMemcpyHostToDevice),
                 "upload tune counts");
       float tmin = -1.0f;
       for (int rep = 0; rep < 4; ++rep) {  // the first run warms up
-        clear_launch_error();
         check_hip(hipEventRecord(e0, stream), "hipEventRecord");
-        hipLaunchKernelGGL(reset_counter, dim3(1), dim3(64), 0, stream, dCounter);
-        if (f64)
-          hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
-                             static_cast<const ModelDesc*>(model_table_), dSpans, dCounts,
-                             dCounter, static_cast<const double*>(dX), dPart);
-        else
-          hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, stream,
-                             static_cast<const ModelDesc*>(model_table_), dSpans, dCounts,
-                             dCounter, static_cast<const float*>(dX), dPart);
-        hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0,
-                           stream, dSpans, dCounts, dPart, dOut);
+        enqueue_kernels(stream, dSpans, dCounts, dCounter, dX, dPart, dOut);
         check_hip(hipGetLastError(), "tune launch");
         check_hip(hipEventRecord(e1, stream), "hipEventRecord");
         check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -1281,6 +1442,139 @@ void Server::tune(Model& m) {
   std::lock_guard<std::mutex> lk(mu_);
   tuned_[tune_key(m.basis, m.D)] = res;
   save_cache();
+}
+
+// Subspaces per work item when the subspace algorithm runs untuned.
+constexpr int kSubDefaultChunks = 32;
+
+// Chooses between the streaming kernel and the subspace kernel for model m
+// (which has a subspace layout) and the subspaces per work item: with the
+// autotuner, both are timed on this model with synthetic uniform queries (the
+// same load as the shape tuner; streaming at its tuned chunk, the subspace
+// kernel at nsub / 2^k subspaces per item) and the faster kept; without it,
+// the configured algorithm decides (auto = streaming) and the subspace kernel
+// takes ceil(nsub / 32) subspaces per item. The caller holds a reference that
+// keeps the model resident.
+void Server::choose_algorithm(Model& m) {
+  const int want = cfg_.algorithm;
+  if (!m.sr || want == kServeStreaming) return;
+  const int untuned = (m.nsub + kSubDefaultChunks - 1) / kSubDefaultChunks;
+  if (!cfg_.autotune) {
+    if (want == kServeSubspace) {
+      m.sub = true;
+      m.sub_chunk = untuned;
+    }
+    return;
+  }
+  const bool f64 = cfg_.precision == Precision::FP64;
+  const size_t esz = f64 ? 8 : 4;
+  const size_t D = static_cast<size_t>(m.D);
+  const size_t nq = kTuneQueries * (cfg_.max_requests_per_batch == 1 ? 1 : kTuneBatchCopies);
+  // queries with low parts (FP32): rows [D, 2D)
+  std::vector<char> hx(nq * D * esz * 2);
+  uint64_t st = 0xC0FFEE00ull + static_cast<uint64_t>(D);
+  for (size_t d = 0; d < D; ++d)
+    for (size_t i = 0; i < nq; ++i) {
+      const double v = static_cast<double>(splitmix64(st) >> 11) * 0x1.0p-53;
+      if (f64) {
+        reinterpret_cast<double*>(hx.data())[d * nq + i] = v;
+      } else {
+        const float hi = static_cast<float>(v);
+        reinterpret_cast<float*>(hx.data())[d * nq + i] = hi;
+        reinterpret_cast<float*>(hx.data())[(D + d) * nq + i] =
+            static_cast<float>(v - static_cast<double>(hi));
+      }
+    }
+  size_t stream_chunk;
+  {
+    std::lock_guard<std::mutex> lk(mu_);
+    stream_chunk = chunk_for(m);
+  }
+  const size_t stile = static_cast<size_t>(kBlock) * serve_k_rt(cfg_.precision, m.D);
+  const size_t s_items = (nq / stile) * ((m.Npad + stream_chunk - 1) / stream_chunk);
+  const size_t partcap = std::max(kTunePartDoubles, s_items * stile);
+  check_hip(hipSetDevice(cfg_.device), "hipSetDevice");
+  hipStream_t stream = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void* dX = nullptr;
+  Span* dSpans = nullptr;
+  int* dCounts = nullptr;
+  unsigned* dCounter = nullptr;
+  double* dPart = nullptr;
+  double* dOut = nullptr;
+  auto release = [&] {
+    if (stream) (void)hipStreamSynchronize(stream);
+    void* p[] = {dX, dSpans, dCounts, dCounter, dPart, dOut};
+    for (void* x : p)
+      if (x) (void)hipFree(x);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+  };
+  try {
+    check_hip(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    check_hip(hipEventCreate(&e0), "hipEventCreate");
+    check_hip(hipEventCreate(&e1), "hipEventCreate");
+    check_hip(hipMalloc(&dX, hx.size()), "hipMalloc(choice X)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dSpans), sizeof(Span)), "hipMalloc(span)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dOut), nq * sizeof(double)), "hipMalloc(out)");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), kCounts * sizeof(int)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), 2 * sizeof(unsigned)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dPart), partcap * sizeof(double)),
+              "hipMalloc(choice partials)");
+    check_hip(hipMemcpy(dX, hx.data(), hx.size(), hipMemcpyHostToDevice), "upload choice X");
+    // best of three timed runs after a warm-up, kernels and reduction
+    auto time_ms = [&](const Span& sp, const int* counts) {
+      check_hip(hipMemcpy(dSpans, &sp, sizeof(Span), hipMemcpyHostToDevice), "upload span");
+      check_hip(hipMemcpy(dCounts, counts, kCounts * sizeof(int), hipMemcpyHostToDevice),
+                "upload counts");
+      float tmin = -1.0f;
+      for (int rep = 0; rep < 4; ++rep) {
+        check_hip(hipEventRecord(e0, stream), "hipEventRecord");
+        enqueue_kernels(stream, dSpans, dCounts, dCounter, dX, dPart, dOut);
+        check_hip(hipGetLastError(), "choice launch");
+        check_hip(hipEventRecord(e1, stream), "hipEventRecord");
+        check_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+        float ms = 0.0f;
+        check_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+        if (rep > 0 && (tmin < 0.0f || ms < tmin)) tmin = ms;
+      }
+      return static_cast<double>(tmin);
+    };
+    double best = -1.0;
+    if (want != kServeSubspace) {
+      // the streaming kernel at its chunk; its queries' low parts (FP32) are
+      // rows [D, 2D) as here when the model uses them
+      const Span sp = make_span(m.table_index, m.D, m.Npad, stile, stream_chunk, nq, nq, 0, 0, 0,
+                                0, 0, m.lo);
+      const int counts[kCounts] = {static_cast<int>(s_items), static_cast<int>(nq), 1, 0, 1};
+      best = time_ms(sp, counts);
+    }
+    int best_chunk = 0;
+    for (int k = 1; k <= 256; k *= 2) {
+      const int c = (m.nsub + k - 1) / k;
+      const size_t nchunks = (static_cast<size_t>(m.nsub) + c - 1) / c;
+      if (nq * nchunks > partcap) break;
+      if (k > 1 && c == (m.nsub + k / 2 - 1) / (k / 2)) continue;  // same chunk as before
+      const size_t items = (nq / kBlock) * nchunks;
+      const Span sp = make_span(m.table_index, m.D, static_cast<size_t>(m.nsub), kBlock,
+                                static_cast<size_t>(c), nq, nq, 0, 0, 0, 0, 0, !f64);
+      const int counts[kCounts] = {0, static_cast<int>(nq), 0, static_cast<int>(items), 1};
+      const double t = time_ms(sp, counts);
+      if (best < 0.0 || t < best) {
+        best = t;
+        best_chunk = c;
+      }
+    }
+    if (best_chunk > 0) {
+      m.sub = true;
+      m.sub_chunk = best_chunk;
+    }
+  } catch (...) {
+    release();
+    throw;
+  }
+  release();
 }
 
 // Cache file: one line per tuned shape,

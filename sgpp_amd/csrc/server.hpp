@@ -27,6 +27,12 @@
 //     order, and a query's result depends only on its model and that
 //     chunking, so a result is bit-identical whether the request runs alone
 //     or in any mix;
+//   * a model whose grid has few points per subspace evaluation can instead be
+//     served by a second work-list kernel in the same launch that visits,
+//     per query, each subspace (level vector) once and takes the one basis
+//     function whose support holds the query from a dense index table, in
+//     FP64 (the SubspaceLinear algorithm); with the autotuner the faster of
+//     the two is chosen per model at admission;
 //   * a request larger than a batch's buffers is split by queries into
 //     pieces over successive batches (results unchanged, by the above); only
 //     a model whose single query tile exceeds them fails "over budget".
@@ -52,6 +58,11 @@
 
 namespace sgpp_hip {
 
+// Evaluation algorithm of the served models (ServerConfig::algorithm).
+constexpr int kServeAuto = 0;       // per model, the faster (timed; streaming untuned)
+constexpr int kServeStreaming = 1;  // every (query, grid point) pair
+constexpr int kServeSubspace = 2;   // subspace lookups wherever the model has a layout
+
 struct ServerConfig {
   int device = 0;
   Precision precision = Precision::FP32;  // FP32 products + FP64 accumulation, or FP64
@@ -66,6 +77,7 @@ struct ServerConfig {
   double item_work = 33554432.0;  // untuned: lane-operations per work item (tile x chunk x (2d+1))
   bool autotune = false;          // time tile shapes per (basis, dimension) at admission
   std::string autotune_cache;     // file of tuned shapes, read at start and updated; "" = none
+  int algorithm = kServeAuto;
 };
 
 // Tuned shape of one (basis, dimension): grid points per work item and the
@@ -121,6 +133,8 @@ class Server {
   void inject_kernel_error();
   // the tuned shape of (basis, dim), if tuned or loaded from the cache
   bool tuned_shape(Basis basis, int dim, TunedShape* out);
+  // kServeStreaming or kServeSubspace: the kernel that serves model id
+  int model_algorithm(int id);
   ServerStats stats();
   void drain();  // blocks until no request is queued or running
 
@@ -131,6 +145,10 @@ class Server {
   struct Slot;
   size_t chunk_for(const Model& m) const;
   void tune(Model& m);
+  void choose_algorithm(Model& m);
+  void enqueue_batch_kernels(Slot& s, hipStream_t st);
+  void enqueue_kernels(hipStream_t st, const void* spans, const int* counts, unsigned* counter,
+                       const void* X, double* part, double* out);
   void load_cache();
   void save_cache();
   static int tune_key(Basis basis, int D);

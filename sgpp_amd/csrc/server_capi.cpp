@@ -70,6 +70,7 @@ sgpp_hip_status sgpp_hip_server_create(const sgpp_hip_server_config* c, sgpp_hip
       if (c->item_work > 0) cfg.item_work = c->item_work;
       cfg.autotune = c->autotune != 0;
       if (c->autotune_cache) cfg.autotune_cache = c->autotune_cache;
+      cfg.algorithm = c->algorithm;
     }
     auto* h = new sgpp_hip_server();
     try {
@@ -209,6 +210,17 @@ sgpp_hip_status sgpp_hip_server_tuned_shape(sgpp_hip_server* h, int basis, int d
       out->chunk = t.chunk;
       out->cost = t.cost;
     }
+    return SGPP_HIP_OK;
+  });
+}
+
+sgpp_hip_status sgpp_hip_server_model_algorithm(sgpp_hip_server* h, int model_id,
+                                                int* algorithm) {
+  return guarded([&] {
+    if (!h || !algorithm) throw sgpp_hip::input_error("null argument");
+    *algorithm = h->s->model_algorithm(model_id) == sgpp_hip::kServeSubspace
+                     ? SGPP_HIP_SERVE_SUBSPACE
+                     : SGPP_HIP_SERVE_STREAMING;
     return SGPP_HIP_OK;
   });
 }

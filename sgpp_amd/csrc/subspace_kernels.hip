@@ -44,28 +44,15 @@
 
 #include "cull_device.hpp"
 #include "meval_device.hpp"
+#include "subspace_device.hpp"
 
 namespace sgpp_hip {
 
 namespace {
 
 constexpr int kSubBlock = 256;
-constexpr int kSubMaxD = 20;
-
-// One subspace: offset of its table, the first dimension to recompute, and
-// its levels (a 32-byte record: one wide scalar load).
-struct alignas(16) SubspaceDesc {
-  long long off;
-  int recalc;
-  unsigned char level[kSubMaxD];
-};
-static_assert(sizeof(SubspaceDesc) == 32, "one subspace record is 32 bytes");
-
 // LDS table entries of one multTranspose chunk (32 KiB of FP64)
 constexpr int kSubLds = 4096;
-// Subspace records the kernels load at once (one wait on the scalar loads per
-// batch); the record array carries kSubBatch - 1 records of padding.
-constexpr int kSubBatch = 4;
 
 // A multTranspose chunk: subspaces [s0, s1), table entries [off0, off0 + n);
 // big: one subspace whose table exceeds kSubLds (global atomics)
@@ -76,54 +63,6 @@ struct alignas(16) SubChunk {
   int pad[2];
 };
 static_assert(sizeof(SubChunk) == 32, "one chunk record is 32 bytes");
-
-// Per dimension: the data point's cell in a level-l subspace and the factor
-// of its basis function there. With t = 2^(l-1) x, the support of the
-// function with index i = 2h + 1 is [h, h + 1) in t: h = floor(t) (the table
-// position), u = t - h in [0, 1), and
-//   Linear hat:            min(2u, 2 - 2u)  (1 - |2^l x - i|, exact in FP64)
-//   ModLinear, level 1:    1
-//              h = 0:      2 - 2u           (max(0, 2 - 2^l x) on its cell)
-//              h = last:   2u               (max(0, 2^l x - i + 1))
-//              otherwise:  the hat          (LinearModifiedBasis.hpp:36-53).
-// x = 1 belongs to the last cell (h = 2^(l-1) - 1, u = 1), where the hat is 0
-// and the ModLinear boundary function 2.
-template <bool kMod>
-__device__ __forceinline__ double cell_factor(double x, int l, unsigned& half_index) {
-  const double t = __builtin_ldexp(x, l - 1);  // exact
-  const unsigned last = (1u << (l - 1)) - 1u;
-  const unsigned c = static_cast<unsigned>(t);
-  const unsigned h = c < last ? c : last;
-  half_index = h;
-  const double u = t - static_cast<double>(h);  // exact
-  const double a = u + u;
-  const double b = 2.0 - a;
-  if (!kMod) return __builtin_fmin(a, b);
-  if (l == 1) return 1.0;
-  return h == 0u ? b : (h == last ? a : __builtin_fmin(a, b));
-}
-
-// Dimensions [recalc, D) of subspace sd; those below recalc are shared with
-// the previous subspace.
-template <int D, bool kMod>
-__device__ __forceinline__ void walk_subspace(const double (&x)[D], const SubspaceDesc& sd,
-                                              int recalc, double (&prod)[D + 1],
-                                              unsigned (&flat)[D + 1]) {
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    if (d < recalc) continue;
-    const int l = sd.level[d];
-    if (kMod && l == 1) {
-      prod[d + 1] = prod[d];
-      flat[d + 1] = flat[d];
-      continue;
-    }
-    unsigned h;
-    const double f = cell_factor<kMod>(x[d], l, h);
-    prod[d + 1] = prod[d] * f;
-    flat[d + 1] = (flat[d] << (l - 1)) + h;
-  }
-}
 
 // mult: lanes own data points X[:, m] (stored order), out[m] = r(x_m).
 template <int D, bool kMod>
@@ -333,65 +272,77 @@ using SubDims = SubTable<1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 
 
 }  // namespace
 
-void DeviceMultiEval::set_grid_subspace(const uint32_t* level, const uint32_t* index, size_t N) {
-  if (D_ > static_cast<size_t>(kSubMaxD))
+int64_t subspace_table_limit(size_t N) {
+  return std::max<int64_t>(int64_t(1) << 26, 64 * static_cast<int64_t>(N));
+}
+
+SubspaceLayout build_subspace_layout(size_t D, size_t N, const uint32_t* level,
+                                     const uint32_t* index) {
+  if (D > static_cast<size_t>(kSubMaxD))
     throw input_error("the subspace algorithm supports dimensions up to 20");
   // group the points by level vector; std::map orders level vectors
   // lexicographically, dimension 0 first
   std::map<std::vector<uint8_t>, std::vector<uint32_t>> groups;
-  std::vector<uint8_t> lv(D_);
+  std::vector<uint8_t> lv(D);
   for (size_t j = 0; j < N; ++j) {
-    for (size_t d = 0; d < D_; ++d) {
-      const uint32_t l = level[j * D_ + d], i = index[j * D_ + d];
+    for (size_t d = 0; d < D; ++d) {
+      const uint32_t l = level[j * D + d], i = index[j * D + d];
       if (l < 1 || l > 30 || (i & 1u) == 0 || i >= (1u << l))
         throw input_error("invalid grid point (level/index out of range for a no-boundary grid)");
       lv[d] = static_cast<uint8_t>(l);
     }
     groups[lv].push_back(static_cast<uint32_t>(j));
   }
-  check_hip(hipSetDevice(device_), "hipSetDevice");
-  std::vector<SubspaceDesc> desc;
-  desc.reserve(groups.size());
-  std::vector<int64_t> te(N);
-  int64_t total = 0;
+  SubspaceLayout lay;
+  lay.desc.reserve(groups.size() + kSubBatch - 1);
+  lay.te.resize(N);
   const std::vector<uint8_t>* prev = nullptr;
   for (const auto& g : groups) {
     int levelsum = 0;
-    for (size_t d = 0; d < D_; ++d) levelsum += g.first[d] - 1;
+    for (size_t d = 0; d < D; ++d) levelsum += g.first[d] - 1;
     if (levelsum > 30) throw input_error("a subspace's index table would exceed 2^30 entries");
     SubspaceDesc sd{};
-    sd.off = total;
+    sd.off = lay.total;
     int r = 0;
     if (prev)
-      while (r < static_cast<int>(D_) && (*prev)[r] == g.first[r]) ++r;
+      while (r < static_cast<int>(D) && (*prev)[r] == g.first[r]) ++r;
     sd.recalc = prev ? r : 0;
-    for (size_t d = 0; d < D_; ++d) sd.level[d] = g.first[d];
-    desc.push_back(sd);
-    total += int64_t(1) << levelsum;
+    for (size_t d = 0; d < D; ++d) sd.level[d] = g.first[d];
+    lay.desc.push_back(sd);
+    lay.total += int64_t(1) << levelsum;
     prev = &g.first;
   }
   // a table entry per cell of every subspace; refuse tables far larger than the grid
-  if (total > std::max<int64_t>(int64_t(1) << 26, 64 * static_cast<int64_t>(N)))
+  if (lay.total > subspace_table_limit(N))
     throw input_error("the subspace index tables of this grid would hold more than 64 entries "
                       "per grid point");
-  std::vector<int32_t> ti(static_cast<size_t>(total), -1);
+  lay.ti.assign(static_cast<size_t>(lay.total), -1);
   size_t s = 0;
   for (const auto& g : groups) {
     for (uint32_t j : g.second) {
       int64_t flat = 0;
-      for (size_t d = 0; d < D_; ++d)
-        flat = (flat << (g.first[d] - 1)) + (index[j * D_ + d] >> 1);
-      ti[static_cast<size_t>(desc[s].off + flat)] = static_cast<int32_t>(j);
-      te[j] = desc[s].off + flat;
+      for (size_t d = 0; d < D; ++d) flat = (flat << (g.first[d] - 1)) + (index[j * D + d] >> 1);
+      lay.ti[static_cast<size_t>(lay.desc[s].off + flat)] = static_cast<int32_t>(j);
+      lay.te[j] = lay.desc[s].off + flat;
     }
     ++s;
   }
+  lay.nsub = static_cast<int>(lay.desc.size());
+  lay.desc.resize(lay.desc.size() + kSubBatch - 1, SubspaceDesc{});  // batch padding
+  return lay;
+}
+
+void DeviceMultiEval::set_grid_subspace(const uint32_t* level, const uint32_t* index, size_t N) {
+  check_hip(hipSetDevice(device_), "hipSetDevice");
+  SubspaceLayout lay = build_subspace_layout(D_, N, level, index);
+  std::vector<SubspaceDesc>& desc = lay.desc;
+  const int64_t total = lay.total;
+  const int nsub = lay.nsub;
   // multTranspose chunks: runs of consecutive subspaces whose tables fit in
   // kSubLds entries; a larger table is a chunk of its own
   std::vector<SubChunk> chunks;
-  for (int q = 0; q < static_cast<int>(desc.size()); ++q) {
-    const int64_t n = (q + 1 < static_cast<int>(desc.size()) ? desc[q + 1].off : total) -
-                      desc[q].off;
+  for (int q = 0; q < nsub; ++q) {
+    const int64_t n = (q + 1 < nsub ? desc[q + 1].off : total) - desc[q].off;
     if (n > kSubLds) {
       chunks.push_back(SubChunk{q, q + 1, desc[q].off, static_cast<int>(n), 1, {0, 0}});
       continue;
@@ -406,7 +357,7 @@ void DeviceMultiEval::set_grid_subspace(const uint32_t* level, const uint32_t* i
   N_ = N;
   Npad_ = N;
   nchunk_ = static_cast<int>(chunks.size());
-  nsub_ = static_cast<int>(desc.size());
+  nsub_ = nsub;
   ntable_ = total;
   // evaluated dimensions per data row: those not shared with the previous
   // subspace (multTranspose: nor at a chunk's start), ModLinear level 1 excluded
@@ -417,20 +368,21 @@ void DeviceMultiEval::set_grid_subspace(const uint32_t* level, const uint32_t* i
     return n;
   };
   sub_recompute_ = sub_recompute_t_ = 0;
-  for (const SubspaceDesc& sd : desc) sub_recompute_ += evaluated(sd, sd.recalc);
+  for (int q = 0; q < nsub; ++q) sub_recompute_ += evaluated(desc[q], desc[q].recalc);
   for (const SubChunk& c : chunks)
     for (int q = c.s0; q < c.s1; ++q)
       sub_recompute_t_ += evaluated(desc[q], q == c.s0 ? 0 : desc[q].recalc);
-  desc.resize(desc.size() + kSubBatch - 1, SubspaceDesc{});  // batch padding
   ensure(&sub_desc_, &sub_desc_cap_, desc.size() * sizeof(SubspaceDesc));
   check_hip(hipMemcpy(sub_desc_, desc.data(), desc.size() * sizeof(SubspaceDesc),
                       hipMemcpyHostToDevice),
             "upload subspaces");
-  ensure(&sub_ti_, &sub_ti_cap_, ti.size() * sizeof(int32_t));
-  check_hip(hipMemcpy(sub_ti_, ti.data(), ti.size() * sizeof(int32_t), hipMemcpyHostToDevice),
+  ensure(&sub_ti_, &sub_ti_cap_, lay.ti.size() * sizeof(int32_t));
+  check_hip(hipMemcpy(sub_ti_, lay.ti.data(), lay.ti.size() * sizeof(int32_t),
+                      hipMemcpyHostToDevice),
             "upload subspace tables");
-  ensure(&sub_te_, &sub_te_cap_, te.size() * sizeof(int64_t));
-  check_hip(hipMemcpy(sub_te_, te.data(), te.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+  ensure(&sub_te_, &sub_te_cap_, lay.te.size() * sizeof(int64_t));
+  check_hip(hipMemcpy(sub_te_, lay.te.data(), lay.te.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice),
             "upload grid point entries");
   ensure(&sub_chunk_, &sub_chunk_cap_, chunks.size() * sizeof(SubChunk));
   check_hip(hipMemcpy(sub_chunk_, chunks.data(), chunks.size() * sizeof(SubChunk),

@@ -5,7 +5,9 @@ error) that never stops the loop.
 Query points are drawn in FP64 and submitted as they are; the FP32 server
 rounds them itself. Tolerances: FP32-with-FP64-accumulation server 1e-5
 relative, FP64 server 1e-12 relative (||got - ref||_2 / ||ref||_2 per
-request)."""
+request). Every behaviour is checked for both evaluation algorithms of the
+server (streaming and subspace lookups); the subspace kernel computes in FP64
+from the queries' FP64 values (hi + lo parts on an FP32 server)."""
 import time
 
 import numpy as np
@@ -48,12 +50,19 @@ def small_server(capi, precision, **kw):
     return capi.Server(precision=precision, **kw)
 
 
+ALGOS = [1, 2]  # SERVE_STREAMING, SERVE_SUBSPACE
+
+
+@pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("prec,tol", [(1, 1e-5), (0, 1e-12)])
-def test_results_match_oracle(gpu, prec, tol):
+def test_results_match_oracle(gpu, prec, tol, algo):
     capi = gpu
-    srv = small_server(capi, prec)
+    srv = small_server(capi, prec, algorithm=algo)
     models = small_models(capi)
     ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    assert all(srv.model_algorithm(m) == algo for m in ids)
+    if algo == capi.SERVE_SUBSPACE:
+        tol = 1e-12  # FP64 lookups on either server
     reqs = []
     for k, q in enumerate([1, 7, 300, 1025, 5000]):
         for m, (b, lv, ix, a) in zip(ids, models):
@@ -71,9 +80,10 @@ def test_results_match_oracle(gpu, prec, tol):
     srv.close()
 
 
-def test_alone_and_in_a_mix_are_identical(gpu):
+@pytest.mark.parametrize("algo", ALGOS)
+def test_alone_and_in_a_mix_are_identical(gpu, algo):
     capi = gpu
-    srv = small_server(capi, 1)
+    srv = small_server(capi, 1, algorithm=algo)
     models = small_models(capi)
     ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
     probes = [(ids[j], workloads.uniform_points(70 + j, q, models[j][1].shape[1]))
@@ -105,17 +115,18 @@ def test_alone_and_in_a_mix_are_identical(gpu):
     srv.close()
 
 
+@pytest.mark.parametrize("algo", ALGOS)
 @pytest.mark.parametrize("prec,tol", [(1, 1e-5), (0, 1e-12)])
-def test_large_requests_are_split(gpu, prec, tol):
+def test_large_requests_are_split(gpu, prec, tol, algo):
     """Requests larger than a batch's buffers (queries, work items, partials)
     run as pieces over several batches, interleaved with other traffic; every
     result is bit-identical to the same request on a server whose batches
     hold it whole, and within tolerance of the oracle."""
     capi = gpu
     models = small_models(capi)
-    whole = small_server(capi, prec)
+    whole = small_server(capi, prec, algorithm=algo)
     split = small_server(capi, prec, max_batch_queries=1 << 12, max_items=64,
-                         max_partial_doubles=1 << 16)
+                         max_partial_doubles=1 << 16, algorithm=algo)
     ids_w = [whole.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
     ids_s = [split.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
     reqs = []
@@ -151,6 +162,21 @@ def test_autotune_shapes_and_cache(gpu, tmp_path):
     srv = small_server(capi, 1, autotune=True, autotune_cache=cache)
     models = small_models(capi)
     ids = [srv.admit_model(b, lv, ix, a) for b, lv, ix, a in models]
+    # with the tuner the faster kernel serves each model: a regular Linear
+    # d=4 level-9 grid (N = 65,537 points in 495 subspaces) is far cheaper by
+    # subspace lookups
+    big = capi.Grid.linear(4).regular(9)
+    blv, bix = big.level_index()
+    balpha = workloads.normal(workloads.stream(99, 0), big.size)
+    bid = srv.admit_model(capi.BASIS_LINEAR, blv, bix, balpha)
+    assert srv.model_algorithm(bid) == capi.SERVE_SUBSPACE
+    Xb = workloads.uniform_points(98, 3000, 4)
+    rid, n = srv.submit(bid, Xb)
+    st, r, _ = srv.poll(rid, n)
+    assert st == capi.REQ_DONE
+    assert rel(r, oracle.mult(0, Xb, blv, bix, balpha)) <= 1e-12
+    assert all(srv.model_algorithm(m) in (capi.SERVE_STREAMING, capi.SERVE_SUBSPACE)
+               for m in ids)
     shapes = {}
     for b, lv, ix, a in models:
         sh = srv.tuned_shape(b, lv.shape[1])
@@ -166,7 +192,7 @@ def test_autotune_shapes_and_cache(gpu, tmp_path):
             assert rel(r, oracle.mult(b, X, lv, ix, a)) <= 1e-5
     srv.close()
     with open(cache) as f:
-        assert f.read().count("sgpp_hip_server_shape") == len(models)
+        assert f.read().count("sgpp_hip_server_shape") == len(models) + 1
     srv2 = small_server(capi, 1, autotune=True, autotune_cache=cache)
     for (b, d), sh in shapes.items():
         got = srv2.tuned_shape(b, d)
@@ -419,3 +445,43 @@ def test_poll_refuses_a_short_output_buffer(gpu):
     with pytest.raises(ValueError):
         srv.submit(mid, X[0])
     srv.close()
+
+
+def test_served_workload_on_the_subspace_kernel(gpu):
+    """The 64 models of the served workload on a server that evaluates every
+    model by subspace lookups (FP64) against a streaming FP64 server, over
+    the first 0.1 s of the request stream: every request within 1e-12 of the
+    streaming result; sampled requests against the oracle."""
+    capi = gpu
+    c = workloads.config_serving(duration=0.1)
+    dims = [m["dim"] for m in c["models"]]
+    sub = capi.Server(precision=0, algorithm=capi.SERVE_SUBSPACE)
+    ref = capi.Server(precision=0, algorithm=capi.SERVE_STREAMING)
+    ids = {}
+    for name, srv in (("sub", sub), ("ref", ref)):
+        ids[name] = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
+                     for m in c["models"]]
+    n_sub = sum(sub.model_algorithm(i) == capi.SERVE_SUBSPACE for i in ids["sub"])
+    print("models on the subspace kernel: %d of %d" % (n_sub, len(ids["sub"])), flush=True)
+    assert n_sub >= len(ids["sub"]) - 4  # only grids with oversized tables stay streaming
+    reqs = []
+    for k in range(len(c["q"])):
+        m, q, off = int(c["model"][k]), int(c["q"][k]), int(c["offset"][k])
+        X = c["pools"][dims[m]][off:off + q]
+        reqs.append((k, m, X, sub.submit(ids["sub"][m], X), ref.submit(ids["ref"][m], X)))
+    worst = 0.0
+    for k, m, X, (rs, n), (rr, _) in reqs:
+        st, a, _ = sub.poll(rs, n)
+        assert st == capi.REQ_DONE, (k, capi.REQ_NAMES[st])
+        st, b, _ = ref.poll(rr, n)
+        assert st == capi.REQ_DONE
+        e = rel(a, b)
+        worst = max(worst, e)
+        assert e <= 1e-12, (k, m, e)
+        if k % 32 == 0:
+            mm = c["models"][m]
+            o = oracle.mult(mm["basis"], X[:64], mm["level"], mm["index"], mm["alpha"])
+            assert rel(a[:64], o) <= 1e-12
+    print("requests %d, worst relative difference %.2e" % (len(reqs), worst), flush=True)
+    sub.close()
+    ref.close()
