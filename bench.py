@@ -53,6 +53,9 @@ def parse():
                     help="serve with the untuned default tile shapes")
     ap.add_argument("--serve-rate", type=float, default=2000.0,
                     help="Poisson arrival rate of the served workload (requests/s)")
+    ap.add_argument("--serve-baseline-kernel-choice", action="store_true",
+                    help="also run the one-launch baseline with the served loop's per-model "
+                         "kernel choice")
     ap.add_argument("--serve-no-baseline", action="store_true",
                     help="serve only; skip the one-launch-per-request FP64 baseline")
     ap.add_argument("--serve-autotune-cache", default=None,
@@ -280,12 +283,15 @@ def thin_stream(c, frac, seed=12):
     return out
 
 
-def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
+def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None, baseline_auto=False):
     """The served workload (64 resident models, Poisson requests) at each rate
     of `rates` for `seconds`: one stream is drawn at the highest rate and the
-    lower rates are thinned from it. The FP32 server (continuous batching)
-    and the baseline (one launch per request, plain FP64, same resident
-    models) each admit the models once and replay every stream in real time.
+    lower rates are thinned from it. The FP32 server (continuous batching;
+    with the tuner, each model on the faster of the streaming and the
+    subspace kernel) and the baseline (one launch per request, the plain FP64
+    streaming kernel, same resident models) each admit the models once and
+    replay every stream in real time. baseline_auto adds the one-launch
+    baseline with the same per-model kernel choice as the served loop.
     Returns (per-rate results, the stream at the highest rate, outputs)."""
     from sgpp_amd import capi, workloads
     rates = sorted(rates)
@@ -297,7 +303,12 @@ def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
     if baseline:
         configs.append(("one_launch_per_request_fp64",
                         dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
-                             use_graph=False)))
+                             use_graph=False, algorithm=capi.SERVE_STREAMING)))
+    if baseline_auto:
+        configs.append(("one_launch_per_request_fp64_kernel_choice",
+                        dict(precision=capi.PREC_FP64, max_inflight=1, max_requests_per_batch=1,
+                             use_graph=False, algorithm=capi.SERVE_AUTO)))
+    algorithms = {}
     runs = {r: {} for r in rates}
     outputs = {r: {} for r in rates}
     for name, kw in configs:
@@ -308,8 +319,10 @@ def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
         ids = [srv.admit_model(m["basis"], m["level"], m["index"], m["alpha"])
                for m in c_hi["models"]]
         admit_s = time.perf_counter() - t
-        print("[serve] %s: %d models admitted in %.1f s" % (name, len(ids), admit_s),
-              file=sys.stderr, flush=True)
+        n_sub = sum(srv.model_algorithm(i) == capi.SERVE_SUBSPACE for i in ids)
+        algorithms[name] = {"subspace": n_sub, "streaming": len(ids) - n_sub}
+        print("[serve] %s: %d models admitted in %.1f s (%d on the subspace kernel)"
+              % (name, len(ids), admit_s, n_sub), file=sys.stderr, flush=True)
         for r in rates:
             res, outs = drive_server(srv, ids, streams[r])
             res["admit_and_tune_s"] = admit_s
@@ -327,7 +340,7 @@ def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
     N = np.array([m["level"].shape[0] for m in c_hi["models"]])
     D = np.array([m["dim"] for m in c_hi["models"]])
     out = {"generation_s": gen_s, "seconds": seconds, "tuned_shapes": runs.pop("tuned_shapes", None),
-           "rates": {}}
+           "models_per_kernel": algorithms, "rates": {}}
     for r in rates:
         c = streams[r]
         flops = float(sum(6 * D[m] * N[m] * q for m, q in zip(c["model"], c["q"])))
@@ -343,6 +356,7 @@ def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
             "offered_evals_per_s": float(c["q"].sum() / seconds),
             "offered_tflops_6d": flops / seconds / 1e12,
             "served": sv, "baseline": base,
+            "baseline_kernel_choice": runs[r].get("one_launch_per_request_fp64_kernel_choice"),
             "evals_ratio_vs_baseline": sv["evals_per_s"] / base["evals_per_s"] if base else None,
             "p99_ratio_vs_baseline": sv["p99_ms"] / base["p99_ms"] if base else None,
             "max_rel_err_vs_fp64": float(max(errs)) if errs else None,
@@ -353,7 +367,8 @@ def serve_leg(rates, seconds, baseline=True, autotune=True, cache=None):
 def serve_bench(args):
     rates = [args.serve_rate]
     leg, c = serve_leg(rates, args.serve_seconds, baseline=not args.serve_no_baseline,
-                       autotune=not args.no_autotune, cache=args.serve_autotune_cache)
+                       autotune=not args.no_autotune, cache=args.serve_autotune_cache,
+                       baseline_auto=args.serve_baseline_kernel_choice)
     r = leg["rates"]["%d" % int(args.serve_rate)]
     N = [m["level"].shape[0] for m in c["models"]]
     line = {
@@ -364,7 +379,7 @@ def serve_bench(args):
         "higher_is_better": True,
         "p50_ms": r["served"]["p50_ms"],
         "p99_ms": r["served"]["p99_ms"],
-        "dtype": "fp32 products, fp64 accumulation",
+        "dtype": "fp32 products with fp64 accumulation (streaming kernel), fp64 (subspace kernel)",
         "data": "synthetic, seeded: 64 models, Poisson %.0f req/s for %.1f s" % (args.serve_rate,
                                                                               args.serve_seconds),
         "config": {"workload": "served_64_models_poisson_%dreqs" % int(args.serve_rate),
@@ -374,6 +389,8 @@ def serve_bench(args):
                    "offered_tflops_6d": r["offered_tflops_6d"]},
         "served": r["served"],
         "baseline": r["baseline"],
+        "baseline_kernel_choice": r["baseline_kernel_choice"],
+        "models_per_kernel": leg["models_per_kernel"],
         "evals_ratio_vs_baseline": r["evals_ratio_vs_baseline"],
         "p99_ratio_vs_baseline": r["p99_ratio_vs_baseline"],
         "max_rel_err_vs_fp64": r["max_rel_err_vs_fp64"],
