@@ -73,8 +73,10 @@ struct Span {
 static_assert(sizeof(Span) == 64, "one span is 64 bytes");
 
 // Subspace-algorithm spans: the same layout, with `chunk` subspaces per
-// item, `nchunks` = ceil(nsub / chunk) and one query per lane (tile = kBlock);
-// their items are numbered apart from the streaming items (counts below).
+// item, `nchunks` = ceil(nsub / chunk) and one query per lane (tile = kBlock;
+// two per lane at d <= 10 spilled registers and served no more);
+// their items are numbered apart from the streaming items and follow them in
+// the work list (counts below).
 // counts[0] = streaming items, [1] = reduction slots, [2] = streaming spans
 // (spans [0, counts[2])), [3] = subspace items, [4] = all spans (the subspace
 // spans are [counts[2], counts[4])).
@@ -330,58 +332,10 @@ __device__ __forceinline__ void dispatch_one(const Item& it, const ModelDesc& md
   serve_item<D, serve_k<T>(D), serve_j<T>(D), T>(it, md, X, part, sbuf, stage_bytes<T>(kMaxD));
 }
 
-// Persistent work-list kernel: blocks pull items from a counter until the
-// list is exhausted (every block reaches the exit test). counts[0] = items,
-// counts[2] = spans. FP32 is held to 128 registers (4 waves per SIMD); the
-// FP64 dimension cases need more and run 3 waves per SIMD.
-template <typename T>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1)))
-void serve_kernel(const ModelDesc* __restrict__ models, const Span* __restrict__ spans,
-                  const int* __restrict__ counts, unsigned* counter, const T* __restrict__ X,
-                  double* __restrict__ part) {
-  // two staging buffers (double-buffered grid tiles)
-  __shared__ __align__(16) unsigned char sbuf[2 * stage_bytes<T>(kMaxD)];
-  __shared__ unsigned s_item;
-  const int n = counts[0];
-  const int nspans = counts[2];
-  for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
-    __syncthreads();
-    const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(s_item));
-    __syncthreads();
-    if (it >= n || it < 0) break;
-    const Span sp = spans[find_span(nspans, it, [&](int k) { return spans[k].item0; })];
-    const ModelDesc md = models[sp.model];
-    const int local = it - sp.item0;
-    const int t = local / sp.nchunks;
-    Item item;
-    item.q0 = t * sp.tile;
-    item.j0 = (local - t * sp.nchunks) * sp.chunk;
-    item.j1 = min(item.j0 + sp.chunk, md.Npad);
-    item.ldq = sp.ldq;
-    item.lo = sp.lo;
-    item.xoff = sp.xoff;
-    item.poff = sp.pbase + static_cast<long long>(local) * sp.tile;
-    switch (sp.D) {
-#define SGPP_CASE(DD) \
-  case DD:            \
-    dispatch_one<T, DD>(item, md, X, part, sbuf); \
-    break;
-      SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
-      SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
-      SGPP_CASE(14) SGPP_CASE(15) SGPP_CASE(16) SGPP_CASE(17) SGPP_CASE(18) SGPP_CASE(19)
-      SGPP_CASE(20)
-#undef SGPP_CASE
-      default:
-        break;
-    }
-  }
-}
-
 // One subspace item: queries [q0, q0 + kBlock) of a span (one per lane)
 // against subspaces [sb, se) of its model, in FP64: x = hi (+ lo for FP32
 // queries), per subspace the changed dimensions, the weight gathered from the
-// table. Records are loaded kSubBatch at a time; the first subspace of the
+// table (the next record prefetched); the first subspace of the
 // item starts from scratch. Padding queries (zero coordinates) give values
 // the reduction ignores.
 template <int D, bool kMod, typename T>
@@ -400,10 +354,33 @@ __device__ void serve_sub_item(const ModelDesc& md, const T* __restrict__ X, lon
   prod[0] = 1.0;
   flat[0] = 0u;
   double acc = 0.0;
+  // Records are read as eight words (one wide scalar load, constant address
+  // space). The next record's load is issued once the current record has
+  // arrived (the empty asm uses all of it, and the scheduling barrier keeps
+  // the load below), so it overlaps this subspace's work; record se exists
+  // (padding). The weights are gathered through a global-address-space
+  // pointer (global rather than flat loads).
+  typedef const __attribute__((address_space(4))) unsigned* RecPtr;
+  typedef const __attribute__((address_space(1))) double* WPtr;
+  const RecPtr R = (RecPtr)(md.SR);
+  const WPtr TW = (WPtr)(md.TW);
+  static_assert(sizeof(SubspaceDesc) == 8 * sizeof(unsigned), "eight words per record");
+  unsigned cur[8];
+#pragma unroll
+  for (int w = 0; w < 8; ++w) cur[w] = R[8 * sb + w];
   for (int s = sb; s < se; ++s) {
-    const SubspaceDesc sd = md.SR[s];
+    asm volatile("" ::"s"(cur[0]), "s"(cur[1]), "s"(cur[2]), "s"(cur[3]), "s"(cur[4]),
+                 "s"(cur[5]), "s"(cur[6]), "s"(cur[7]));
+    __builtin_amdgcn_sched_barrier(0);
+    unsigned nxt[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) nxt[w] = R[8 * (s + 1) + w];
+    SubspaceDesc sd;
+    __builtin_memcpy(&sd, cur, sizeof(SubspaceDesc));
     walk_subspace<D, kMod>(x, sd, s == sb ? 0 : sd.recalc, prod, flat);
-    acc = __builtin_fma(md.TW[sd.off + flat[D]], prod[D], acc);
+    acc = __builtin_fma(TW[sd.off + flat[D]], prod[D], acc);
+#pragma unroll
+    for (int w = 0; w < 8; ++w) cur[w] = nxt[w];
   }
   out[threadIdx.x] = acc;
 }
@@ -417,37 +394,74 @@ __device__ __forceinline__ void dispatch_sub(const ModelDesc& md, const T* X, co
     serve_sub_item<D, false, T>(md, X, sp.xoff, sp.ldq, sp.lo, q0, sb, se, out);
 }
 
-// Persistent work-list kernel of the subspace items (counts[3] items of the
-// spans [counts[2], counts[4])), pulling from counter[1].
+// Persistent work-list kernel: blocks pull items from a counter until the
+// list is exhausted (every block reaches the exit test). Items [0, counts[0])
+// are streaming items of the spans [0, counts[2]); items from counts[0] on are
+// the counts[3] subspace items of the spans [counts[2], counts[4]). One kernel
+// runs both kinds, so one tail drains the batch. FP32 is held to 128
+// registers (4 waves per SIMD); the FP64 dimension cases need more and are
+// held to 168 (3 waves per SIMD).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void serve_sub_kernel(const ModelDesc* __restrict__ models,
-                                                           const Span* __restrict__ spans,
-                                                           const int* __restrict__ counts,
-                                                           unsigned* counter,
-                                                           const T* __restrict__ X,
-                                                           double* __restrict__ part) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 3)))
+void serve_kernel(const ModelDesc* __restrict__ models, const Span* __restrict__ spans,
+                  const int* __restrict__ counts, unsigned* counter, const T* __restrict__ X,
+                  double* __restrict__ part) {
+  // two staging buffers (double-buffered grid tiles)
+  __shared__ __align__(16) unsigned char sbuf[2 * stage_bytes<T>(kMaxD)];
   __shared__ unsigned s_item;
-  const int n = counts[3];
+  const int ns = counts[0];
+  const int n = ns + counts[3];
+  const int nspans = counts[2];
   const Span* sspans = spans + counts[2];
-  const int nspans = counts[4] - counts[2];
+  const int nsubspans = counts[4] - counts[2];
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(counter + 1, 1u);
+    if (threadIdx.x == 0) s_item = atomicAdd(counter, 1u);
     __syncthreads();
     const int it = __builtin_amdgcn_readfirstlane(static_cast<int>(s_item));
     __syncthreads();
     if (it >= n || it < 0) break;
-    const Span sp = sspans[find_span(nspans, it, [&](int k) { return sspans[k].item0; })];
-    const ModelDesc md = models[sp.model];
-    const int local = it - sp.item0;
-    const int t = local / sp.nchunks;
-    const int c = local - t * sp.nchunks;
-    const int sb = c * sp.chunk;
-    const int se = min(sb + sp.chunk, md.nsub);
-    double* out = part + sp.pbase + static_cast<long long>(local) * sp.tile;
-    switch (sp.D) {
+    if (it >= ns) {
+      // subspace item
+      const int is = it - ns;
+      const Span sp = sspans[find_span(nsubspans, is, [&](int k) { return sspans[k].item0; })];
+      const ModelDesc md = models[sp.model];
+      const int local = is - sp.item0;
+      const int t = local / sp.nchunks;
+      const int c = local - t * sp.nchunks;
+      const int sb = c * sp.chunk;
+      const int se = min(sb + sp.chunk, md.nsub);
+      double* out = part + sp.pbase + static_cast<long long>(local) * sp.tile;
+      switch (sp.D) {
 #define SGPP_CASE(DD) \
   case DD:            \
     dispatch_sub<DD, T>(md, X, sp, t * sp.tile, sb, se, out); \
+    break;
+        SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
+        SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
+        SGPP_CASE(14) SGPP_CASE(15) SGPP_CASE(16) SGPP_CASE(17) SGPP_CASE(18) SGPP_CASE(19)
+        SGPP_CASE(20)
+#undef SGPP_CASE
+        default:
+          break;
+      }
+      continue;
+    }
+    const Span sp = spans[find_span(nspans, it, [&](int k) { return spans[k].item0; })];
+    const ModelDesc md = models[sp.model];
+    const int local = it - sp.item0;
+    const int t = local / sp.nchunks;
+    Item item;
+    item.q0 = t * sp.tile;
+    item.j0 = (local - t * sp.nchunks) * sp.chunk;
+    item.j1 = min(item.j0 + sp.chunk, md.Npad);
+    item.ldq = sp.ldq;
+    item.lo = sp.lo;
+    item.xoff = sp.xoff;
+    item.poff = sp.pbase + static_cast<long long>(local) * sp.tile;
+    switch (sp.D) {
+#define SGPP_CASE(DD) \
+  case DD:            \
+    dispatch_one<T, DD>(item, md, X, part, sbuf); \
     break;
       SGPP_CASE(1) SGPP_CASE(2) SGPP_CASE(3) SGPP_CASE(4) SGPP_CASE(5) SGPP_CASE(6) SGPP_CASE(7)
       SGPP_CASE(8) SGPP_CASE(9) SGPP_CASE(10) SGPP_CASE(11) SGPP_CASE(12) SGPP_CASE(13)
@@ -494,9 +508,8 @@ __global__ __launch_bounds__(kBlock) void serve_reduce(const Span* __restrict__ 
   }
 }
 
-// both work-list counters (streaming items, subspace items)
 __global__ void reset_counter(unsigned* counter) {
-  if (threadIdx.x < 2 && blockIdx.x == 0) counter[threadIdx.x] = 0u;
+  if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
 }
 
 // Span of q queries of one model (stored [D][ldq] at xoff, ldq a multiple of
@@ -624,8 +637,8 @@ struct Server::Slot {
   size_t nSubItems = 0, nStreamSpans = 0;  // subspace items; spans of the streaming kernel
 };
 
-// reset, streaming work-list kernel, subspace work-list kernel, reduction of
-// slot s's batch on stream st (also what the slot's graph holds)
+// reset, work-list kernel, reduction of slot s's batch on stream st (also
+// what the slot's graph holds)
 void Server::enqueue_batch_kernels(Slot& s, hipStream_t st) {
   enqueue_kernels(st, s.dSpans, s.dCounts, s.dCounter, s.dX, s.dPart, s.dOut);
 }
@@ -639,13 +652,9 @@ void Server::enqueue_kernels(hipStream_t st, const void* spans, const int* count
   if (cfg_.precision == Precision::FP64) {
     hipLaunchKernelGGL(serve_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models, sp,
                        counts, counter, static_cast<const double*>(X), part);
-    hipLaunchKernelGGL(serve_sub_kernel<double>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models,
-                       sp, counts, counter, static_cast<const double*>(X), part);
   } else {
     hipLaunchKernelGGL(serve_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models, sp,
                        counts, counter, static_cast<const float*>(X), part);
-    hipLaunchKernelGGL(serve_sub_kernel<float>, dim3(cfg_.blocks), dim3(kBlock), 0, st, models,
-                       sp, counts, counter, static_cast<const float*>(X), part);
   }
   hipLaunchKernelGGL(serve_reduce, dim3(kReduceBlocksPerCU * num_cus_), dim3(kBlock), 0, st, sp,
                      counts, part, out);
@@ -686,7 +695,7 @@ Server::Server(const ServerConfig& cfg) : cfg_(cfg) {
               "hipMalloc(spans)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounts), kCounts * sizeof(int)),
               "hipMalloc(counts)");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounter), 2 * sizeof(unsigned)),
+    check_hip(hipMalloc(reinterpret_cast<void**>(&s->dCounter), sizeof(unsigned)),
               "hipMalloc(counter)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&s->dPart), s->capPart * sizeof(double)),
               "hipMalloc(partials)");
@@ -1401,7 +1410,7 @@ void Server::tune(Model& m) {
     check_hip(hipMalloc(reinterpret_cast<void**>(&dOut), nq * sizeof(double)),
               "hipMalloc(tune out)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), kCounts * sizeof(int)), "hipMalloc");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), 2 * sizeof(unsigned)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), sizeof(unsigned)), "hipMalloc");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dPart),
                         std::max(kTunePartDoubles, ntiles * tile) * sizeof(double)),
               "hipMalloc(tune partials)");
@@ -1519,7 +1528,7 @@ void Server::choose_algorithm(Model& m) {
     check_hip(hipMalloc(reinterpret_cast<void**>(&dSpans), sizeof(Span)), "hipMalloc(span)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dOut), nq * sizeof(double)), "hipMalloc(out)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dCounts), kCounts * sizeof(int)), "hipMalloc");
-    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), 2 * sizeof(unsigned)), "hipMalloc");
+    check_hip(hipMalloc(reinterpret_cast<void**>(&dCounter), sizeof(unsigned)), "hipMalloc");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dPart), partcap * sizeof(double)),
               "hipMalloc(choice partials)");
     check_hip(hipMemcpy(dX, hx.data(), hx.size(), hipMemcpyHostToDevice), "upload choice X");
