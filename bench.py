@@ -183,9 +183,12 @@ def cpu_baseline(basis, X, grid, lv, ix, alpha, y, seconds):
     }
 
 
-def drive_server(srv, ids, c, warm=32):
-    """Replay the request stream in real time against one server; returns
-    per-request latency (arrival -> result), completion times and outputs."""
+def drive_server(srv, ids, c, warm=32, clients=4):
+    """Replay the request stream in real time against one server, submitted
+    by `clients` threads (request k by thread k mod clients, each at its
+    arrival time; the C-ABI submit is thread-safe and converts the queries on
+    the calling thread); returns per-request latency (arrival -> result),
+    completion times and outputs."""
     import queue
     import threading
 
@@ -204,8 +207,8 @@ def drive_server(srv, ids, c, warm=32):
     pend = queue.Queue()
     t0 = time.perf_counter() + 0.05
 
-    def submitter():
-        for k in range(n):
+    def submitter(j):
+        for k in range(j, n, clients):
             ta = t0 + float(c["arrival"][k])
             now = time.perf_counter()
             if ta > now:
@@ -217,19 +220,25 @@ def drive_server(srv, ids, c, warm=32):
             pend.put((k, rid, q, lag))
         pend.put(None)
 
-    th = threading.Thread(target=submitter)
-    th.start()
-    while True:
+    ths = [threading.Thread(target=submitter, args=(j,)) for j in range(clients)]
+    for th in ths:
+        th.start()
+    ended = 0
+    lags = np.zeros(n)
+    while ended < clients:
         item = pend.get()
         if item is None:
-            break
+            ended += 1
+            continue
         k, rid, q, lag = item
         st, out, l = srv.poll(rid, q, wait_s=-1.0)
         status[k] = st
         lat[k] = lag + l
+        lags[k] = lag
         if k in keep and out is not None:
             outs[k] = out
-    th.join()
+    for th in ths:
+        th.join()
     done = status == 2
     t_complete = c["arrival"] + lat
     span = float(np.nanmax(t_complete[done]) - c["arrival"][0])
@@ -238,7 +247,9 @@ def drive_server(srv, ids, c, warm=32):
                 p50_ms=float(np.percentile(lat[done], 50) * 1e3),
                 p99_ms=float(np.percentile(lat[done], 99) * 1e3),
                 mean_ms=float(np.mean(lat[done]) * 1e3), span_s=span,
-                stats=srv.stats()), outs
+                submit_lag_ms={"mean": float(np.mean(lags) * 1e3),
+                               "max": float(np.max(lags) * 1e3)},
+                clients=clients, stats=srv.stats()), outs
 
 
 def operator_per_request(c, n):

@@ -1003,25 +1003,39 @@ uint64_t Server::submit(int model, size_t q, const double* queries, double deadl
   r->q = q;
   r->out.assign(q, 0.0);
   const size_t D = m->D;
-  for (size_t i = 0; i < q * D; ++i)
-    if (!(queries[i] >= 0.0 && queries[i] <= 1.0))
-      throw input_error("query points must lie in [0,1]^d");
   const bool f64 = cfg_.precision == Precision::FP64;
-  // FP32: hi = fl32(x) and, for models that add them, lo = fl32(x - hi)
-  r->X.resize(D * q * (f64 ? 8 : (m->lo ? 8 : 4)));
-  for (size_t i = 0; i < q; ++i)
+  const bool lo = !f64 && m->lo;
+  // [D][q] in the compute type; FP32: hi = fl32(x) and, for models that add
+  // them, lo = fl32(x - hi) in rows [D, 2D). Transposed in blocks of kCvt
+  // queries (each output row written contiguously), checked on the way.
+  r->X.resize(D * q * (f64 ? 8 : (lo ? 8 : 4)));
+  constexpr size_t kCvt = 64;
+  bool in_cube = true;
+  for (size_t i0 = 0; i0 < q; i0 += kCvt) {
+    const size_t n = std::min(kCvt, q - i0);
     for (size_t d = 0; d < D; ++d) {
-      const double v = queries[i * D + d];
+      const double* src = queries + i0 * D + d;
       if (f64) {
-        reinterpret_cast<double*>(r->X.data())[d * q + i] = v;
+        double* h = reinterpret_cast<double*>(r->X.data()) + d * q + i0;
+        for (size_t k = 0; k < n; ++k) {
+          const double v = src[k * D];
+          in_cube &= (v >= 0.0) & (v <= 1.0);
+          h[k] = v;
+        }
       } else {
-        const float hi = static_cast<float>(v);
-        reinterpret_cast<float*>(r->X.data())[d * q + i] = hi;
-        if (m->lo)
-          reinterpret_cast<float*>(r->X.data())[(D + d) * q + i] =
-              static_cast<float>(v - static_cast<double>(hi));
+        float* h = reinterpret_cast<float*>(r->X.data()) + d * q + i0;
+        float* l = lo ? reinterpret_cast<float*>(r->X.data()) + (D + d) * q + i0 : nullptr;
+        for (size_t k = 0; k < n; ++k) {
+          const double v = src[k * D];
+          in_cube &= (v >= 0.0) & (v <= 1.0);
+          const float hv = static_cast<float>(v);
+          h[k] = hv;
+          if (lo) l[k] = static_cast<float>(v - static_cast<double>(hv));
+        }
       }
     }
+  }
+  if (!in_cube) throw input_error("query points must lie in [0,1]^d");
   r->t_submit = t_enter;  // latency includes the conversion of the queries
   r->deadline = deadline_s > 0 ? r->t_submit + deadline_s : 0.0;
   {
@@ -1455,6 +1469,8 @@ void Server::tune(Model& m) {
 
 // Subspaces per work item when the subspace algorithm runs untuned.
 constexpr int kSubDefaultChunks = 32;
+// synthetic load of the algorithm choice on a batching server (x kTuneQueries)
+constexpr size_t kChoiceBatchCopies = 16;
 
 // Chooses between the streaming kernel and the subspace kernel for model m
 // (which has a subspace layout) and the subspaces per work item: with the
@@ -1478,7 +1494,10 @@ void Server::choose_algorithm(Model& m) {
   const bool f64 = cfg_.precision == Precision::FP64;
   const size_t esz = f64 ? 8 : 4;
   const size_t D = static_cast<size_t>(m.D);
-  const size_t nq = kTuneQueries * (cfg_.max_requests_per_batch == 1 ? 1 : kTuneBatchCopies);
+  // a lone request when every launch holds one; otherwise enough queries that
+  // even one subspace chunk per query tile fills the device, as the many
+  // models of a saturated batch do together
+  const size_t nq = kTuneQueries * (cfg_.max_requests_per_batch == 1 ? 1 : kChoiceBatchCopies);
   // queries with low parts (FP32): rows [D, 2D)
   std::vector<char> hx(nq * D * esz * 2);
   uint64_t st = 0xC0FFEE00ull + static_cast<uint64_t>(D);
