@@ -76,7 +76,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true",
                     help="skip the served-loop leg of the default run")
     ap.add_argument("--serve-leg-seconds", type=float, default=1.5,
-                    help="seconds of each request stream (2000 and 8000 req/s) in the default run")
+                    help="seconds of each request stream (2000 and 16000 req/s) in the default run")
     ap.add_argument("--grid-target", type=int, default=100000)
     ap.add_argument("--data-points", type=int, default=1000000)
     return ap.parse_args()
@@ -779,7 +779,7 @@ def main():
                                             args.cpu_seconds)
     del op, d_alpha, d_y, d_r, d_a
     if not args.no_serve:
-        leg, _ = serve_leg([2000.0, 8000.0], args.serve_leg_seconds)
+        leg, _ = serve_leg([2000.0, 16000.0], args.serve_leg_seconds)
         line["served"] = leg
     print(json.dumps(line))
 
