@@ -53,8 +53,7 @@ __device__ __forceinline__ double dpp(double old, double v) {
 // quad's value), shift right by 4 and 8 within each row of 16 (lane 15 of a
 // row holds the row's value), broadcast lane 15 into the next row (lanes 31
 // and 63 hold two rows each), broadcast lane 31 into rows 2-3 (lane 63).
-// For sums the lanes outside a step's reach add 0; for min/max they combine
-// with themselves.
+// The lanes outside a step's reach add 0.
 template <typename T>
 __device__ __forceinline__ T sum_to_lane63(T v) {
   v += dpp<0xb1>(T(0), v);        // quad_perm [1,0,3,2]
@@ -63,26 +62,6 @@ __device__ __forceinline__ T sum_to_lane63(T v) {
   v += dpp<0x118>(T(0), v);       // row_shr:8
   v += dpp<0x142, 0xa>(T(0), v);  // row_bcast:15 into rows 1, 3
   v += dpp<0x143, 0xc>(T(0), v);  // row_bcast:31 into rows 2, 3
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T min_to_lane63(T v) {
-  v = fmin(v, dpp<0xb1>(v, v));
-  v = fmin(v, dpp<0x4e>(v, v));
-  v = fmin(v, dpp<0x114>(v, v));
-  v = fmin(v, dpp<0x118>(v, v));
-  v = fmin(v, dpp<0x142, 0xa>(v, v));
-  v = fmin(v, dpp<0x143, 0xc>(v, v));
-  return v;
-}
-template <typename T>
-__device__ __forceinline__ T max_to_lane63(T v) {
-  v = fmax(v, dpp<0xb1>(v, v));
-  v = fmax(v, dpp<0x4e>(v, v));
-  v = fmax(v, dpp<0x114>(v, v));
-  v = fmax(v, dpp<0x118>(v, v));
-  v = fmax(v, dpp<0x142, 0xa>(v, v));
-  v = fmax(v, dpp<0x143, 0xc>(v, v));
   return v;
 }
 // lane 63's value, moved to a scalar register
@@ -95,74 +74,38 @@ __device__ __forceinline__ double lane63(double v) {
   const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), 63);
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
-// The box [lo, hi] of a wave's 64*K data points in the first DC dimensions
-// (wave-uniform: scalar registers). Culling tests only these dimensions;
-// testing fewer dimensions keeps more grid points, never drops one that can
-// be nonzero.
+// The box [lo, hi] of a wave's 64*K data points (wave-uniform: scalar
+// registers).
 template <int DC, typename T>
 struct WaveBox {
   T l[DC], h[DC];
 };
 
-template <int DC, int D, int K, typename T>
-__device__ __forceinline__ void wave_box(const T (&x)[K][D], WaveBox<DC, T>& box) {
-  static_assert(DC <= D, "tested dimensions");
+// The box of data group g, computed once when the data are set (the min and
+// max of the group's points per dimension, in T): WB[g][d] = {lo, hi}. g is
+// wave-uniform, so the loads are scalar.
+template <int D, typename T>
+__device__ __forceinline__ void load_box(const T* __restrict__ WB, int64_t g, WaveBox<D, T>& box) {
+  const T* b = WB + g * (D * 2);
 #pragma unroll
-  for (int d = 0; d < DC; ++d) {
-    T a = x[0][d], b = x[0][d];
-#pragma unroll
-    for (int k = 1; k < K; ++k) {
-      a = fmin(a, x[k][d]);
-      b = fmax(b, x[k][d]);
-    }
-    box.l[d] = lane63(min_to_lane63(a));
-    box.h[d] = lane63(max_to_lane63(b));
+  for (int d = 0; d < D; ++d) {
+    box.l[d] = b[2 * d];
+    box.h[d] = b[2 * d + 1];
   }
-}
-
-// False only if the factor max(0, 1 - |L x - I|) (L >= 0) is exactly zero for
-// every x in [lo, hi]: in exact arithmetic that holds when L*hi - I <= -1 or
-// L*lo - I >= 1. fma rounds monotonically, so "fma(L, hi, -I) < -1" implies
-// L*hi - I < -1: a point is only ever dropped when the kernel's own
-// u = fma(L, x, -I) has |u| >= 1 for all x in the box.
-template <typename T>
-__device__ __forceinline__ bool may_hit(T L, T I, T lo, T hi) {
-  return !(fmaT(L, hi, -I) < T(-1)) && !(fmaT(L, lo, -I) > T(1));
 }
 
 // dimensions per batch of bound loads in the tile and point tests
 constexpr int kTestBatch = 8;
 
-// Tile pre-test: the tile's union of supports TB[t][d] = [lo, hi] (FP32,
-// rounded outwards; compared in T, which is exact) against the wave's box.
+// Tile pre-test for up to 64 tiles at once, one tile per lane: bit t of the
+// result is set when tile t0 + t (t < nt <= 64) may hit the box, i.e. when
+// the tile's union of supports TB[t][d] = [lo, hi] (FP32, rounded outwards;
+// compared in T, which is exact) meets the box in every tested dimension.
 // The bounds are loaded in batches of kTestBatch dimensions and all tests of
-// a batch run after its loads (no short-circuit), so the loads are issued
-// together.
-// TB rows hold D dimensions; the first DC are tested.
-template <int DC, int D, typename T>
-__device__ __forceinline__ bool tile_may_hit(const float* __restrict__ TB, int64_t t,
-                                             const WaveBox<DC, T>& box) {
-  const float* tb = TB + t * D * 2;
-  unsigned h = 1u;
-#pragma unroll
-  for (int d0 = 0; d0 < DC; d0 += kTestBatch) {
-    float b[2 * kTestBatch];
-#pragma unroll
-    for (int e = 0; e < 2 * kTestBatch; ++e)
-      if (2 * d0 + e < 2 * DC) b[e] = tb[2 * d0 + e];
-#pragma unroll
-    for (int dd = 0; dd < kTestBatch; ++dd)
-      if (d0 + dd < DC)
-        h &= static_cast<unsigned>(static_cast<T>(b[2 * dd]) < box.h[d0 + dd]) &
-             static_cast<unsigned>(static_cast<T>(b[2 * dd + 1]) > box.l[d0 + dd]);
-  }
-  return h != 0u;
-}
-
-// The same pre-test for up to 64 tiles at once, one tile per lane: bit t of
-// the result is set when tile t0 + t (t < nt <= 64) may hit the box. One
-// vector load round trip for a whole window of tiles instead of one scalar
-// round trip per tile.
+// a batch run after its loads, so the loads are issued together. One vector
+// load round trip covers a whole window of tiles. TB rows hold D dimensions;
+// the first DC are tested (testing fewer keeps more tiles, never drops one
+// that can be nonzero).
 template <int DC, int D, typename T>
 __device__ __forceinline__ unsigned long long tiles_may_hit(const float* __restrict__ TB,
                                                             int64_t t0, int nt, int lane,
@@ -191,12 +134,19 @@ __device__ __forceinline__ unsigned long long tiles_may_hit(const float* __restr
 // (the lane's (L, I) row is read in batches of kTestBatch dimensions).
 // Rows hold D dimensions; the first DC are tested.
 // FP32 rows of up to 12 dimensions are read in one batch (one wait).
+// A point is dropped only if its factor max(0, 1 - |L x - I|) (L >= 0) is
+// exactly zero for every x in the box [lo, hi] in some dimension: in exact
+// arithmetic when L*hi - I <= -1 or L*lo - I >= 1. fma rounds monotonically,
+// so fma(L, hi, -I) < -1 implies L*hi - I < -1, and the kernel's own
+// u = fma(L, x, -I) then has |u| >= 1 for all x in the box. The test over all
+// dimensions is two reductions (min3 / max3 instructions and two compares per
+// point): dropped when min_d fma(L, hi, -I) < -1 or max_d fma(L, lo, -I) > 1.
 template <int DC, int D, typename T>
 __device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ tile, int lane,
                                                         const WaveBox<DC, T>& box) {
   constexpr int B = sizeof(T) == 4 && DC <= 12 ? DC : kTestBatch;
   const T* row = tile + lane * D * 2;
-  unsigned h = 1u;
+  T lo_min = T(0), hi_max = T(0);
 #pragma unroll
   for (int d0 = 0; d0 < DC; d0 += B) {
     T r[2 * B];
@@ -205,10 +155,14 @@ __device__ __forceinline__ unsigned long long tile_hits(const T* __restrict__ ti
       if (2 * d0 + e < 2 * DC) r[e] = row[2 * d0 + e];
 #pragma unroll
     for (int dd = 0; dd < B; ++dd)
-      if (d0 + dd < DC)
-        h &= static_cast<unsigned>(may_hit(r[2 * dd], r[2 * dd + 1], box.l[d0 + dd], box.h[d0 + dd]));
+      if (d0 + dd < DC) {
+        const T a = fmaT(r[2 * dd], box.h[d0 + dd], -r[2 * dd + 1]);
+        const T b = fmaT(r[2 * dd], box.l[d0 + dd], -r[2 * dd + 1]);
+        lo_min = d0 + dd == 0 ? a : fmin(lo_min, a);
+        hi_max = d0 + dd == 0 ? b : fmax(hi_max, b);
+      }
   }
-  return __ballot(h != 0u);
+  return __ballot(!(lo_min < T(-1)) && !(hi_max > T(1)));
 }
 
 // Takes up to J points from the hit mask (wave-uniform). Missing slots

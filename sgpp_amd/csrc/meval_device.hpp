@@ -164,6 +164,13 @@ class DeviceMultiEval {
   template <typename T>
   void run_multT(const T* X, const T* LI, const float* TB, const double* d_y, double* d_result,
                  hipEvent_t* tev);
+  // data-group boxes of the templated kernels (set at construction)
+  template <typename T>
+  void upload_group_boxes(const std::vector<T>& soa, void** dst);
+  template <typename T>
+  const T* group_boxes() const {
+    return static_cast<const T*>(sizeof(T) == sizeof(double) ? WB64_ : WB32_);
+  }
   // the subspace algorithm (subspace_kernels.hip)
   void set_grid_subspace(const uint32_t* level, const uint32_t* index, size_t N);
   void run_subspace_mult(const double* d_alpha, double* d_result, hipEvent_t* tev);
@@ -189,6 +196,11 @@ class DeviceMultiEval {
   // data in storage order, SoA dim-major [D][Mpad]
   void* X64_ = nullptr;
   void* X32_ = nullptr;
+  // per data group (one wave's 64*K consecutive points in the templated
+  // kernels, d <= 20) the box of its points, [Mpad/(64*K)][D][2] {lo, hi}, in
+  // the data's precision
+  void* WB64_ = nullptr;
+  void* WB32_ = nullptr;
   // grid pairs, point-major [Npad][D][2], per-point scale [Npad] and
   // per-tile (64 points) boxes of the supports [Npad/64][D][2], FP32 rounded
   // outwards for both precisions

@@ -222,22 +222,55 @@ DeviceMultiEval::DeviceMultiEval(const double* data, size_t M, size_t dim, Basis
   check_hip(hipMalloc(&dperm_, M_ * sizeof(int32_t)), "hipMalloc(data order)");
   check_hip(hipMemcpy(dperm_, dperm.data(), M_ * sizeof(int32_t), hipMemcpyHostToDevice),
             "upload data order");
+  const bool templ = D_ <= static_cast<size_t>(kMaxTemplD);
   if (serves(Precision::FP64)) {
     check_hip(hipMalloc(&X64_, soa.size() * sizeof(double)), "hipMalloc(X64)");
     check_hip(hipMemcpy(X64_, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice),
               "upload X64");
+    if (templ) upload_group_boxes(soa, &WB64_);
   }
   if (serves(Precision::FP32)) {
     std::vector<float> s32(soa.begin(), soa.end());
     check_hip(hipMalloc(&X32_, s32.size() * sizeof(float)), "hipMalloc(X32)");
     check_hip(hipMemcpy(X32_, s32.data(), s32.size() * sizeof(float), hipMemcpyHostToDevice),
               "upload X32");
+    if (templ) upload_group_boxes(s32, &WB32_);
   }
+}
+
+// Boxes of the data groups of the templated kernels (rows [g*64K, (g+1)*64K)
+// of the stored data; mult and multTranspose use the same K), uploaded to
+// *dst as [Mpad/(64K)][D][2] {lo, hi}. Rounding to FP32 is monotone, so the
+// FP32 boxes are the rounded FP64 ones.
+template <typename T>
+void DeviceMultiEval::upload_group_boxes(const std::vector<T>& soa, void** dst) {
+  const int K = mult_k_rt<T>(static_cast<int>(D_));
+  if (K != multT_k_rt<T>(static_cast<int>(D_)))
+    throw device_error("mult and multTranspose data groups differ");
+  const size_t group = 64 * static_cast<size_t>(K);
+  if (Mpad_ % group != 0) throw device_error("padded data is not a whole number of groups");
+  const size_t ng = Mpad_ / group;
+  std::vector<T> wb(ng * D_ * 2);
+  for (size_t d = 0; d < D_; ++d) {
+    const T* col = soa.data() + d * Mpad_;
+    for (size_t g = 0; g < ng; ++g) {
+      T lo = col[g * group], hi = lo;
+      for (size_t m = g * group + 1; m < (g + 1) * group; ++m) {
+        lo = std::min(lo, col[m]);
+        hi = std::max(hi, col[m]);
+      }
+      wb[(g * D_ + d) * 2] = lo;
+      wb[(g * D_ + d) * 2 + 1] = hi;
+    }
+  }
+  check_hip(hipMalloc(dst, wb.size() * sizeof(T)), "hipMalloc(data group boxes)");
+  check_hip(hipMemcpy(*dst, wb.data(), wb.size() * sizeof(T), hipMemcpyHostToDevice),
+            "upload data group boxes");
 }
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_,     X32_,    LI64_,   LI32_,   TB64_, TB32_, scale_, perm_,
+  void* bufs[] = {X64_,     X32_,    WB64_,   WB32_,   LI64_,   LI32_,   TB64_, TB32_, scale_, perm_,
                   pm_,      dperm_,  A_,      Y_,      part_, hin_,  hout_,  sub_desc_,
                   sub_ti_,  sub_te_, sub_tw_, sub_chunk_};
   for (void* b : bufs)
@@ -403,7 +436,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
     const bool skip = basis_ == Basis::ModLinear;
     MultFn<T> f = count_ ? count_mult_launcher<T>(static_cast<int>(D_), skip)
                          : DimTable<T>::mult(static_cast<int>(D_), skip);
-    f(X, static_cast<int64_t>(Mpad_), LI, TB, static_cast<const uint32_t*>(pm_),
+    f(X, static_cast<int64_t>(Mpad_), group_boxes<T>(), LI, TB, static_cast<const uint32_t*>(pm_),
       static_cast<const T*>(A), P, chunk, jend, G, static_cast<int64_t>(Mpad_), count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(Mpad_ / kBlock), static_cast<unsigned>(G));
@@ -463,7 +496,7 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
     const bool skip = basis_ == Basis::ModLinear;
     MultTFn<T> f = count_ ? count_multT_launcher<T>(static_cast<int>(D_), skip)
                           : DimTable<T>::multT(static_cast<int>(D_), skip);
-    f(X, static_cast<int64_t>(Mpad_), static_cast<const T*>(Y_), LI, TB,
+    f(X, static_cast<int64_t>(Mpad_), group_boxes<T>(), static_cast<const T*>(Y_), LI, TB,
       static_cast<const uint32_t*>(pm_), P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
       count_, s);
   } else {

@@ -40,10 +40,11 @@ constexpr size_t kDataSlack = 1024;
 
 // Points per lane (K) and grid points evaluated together (J). Values from
 // variant sweeps on the d=10 ModLinear, d=8 Linear and d=20 Linear workloads.
-// Kernels for d > 10 are held to 128 registers (4 waves per SIMD) although
-// some values then spill: at d = 20 that was 4-21 % faster than 3 waves
-// without spills; at d = 10 (FP64) it was 10-15 % slower, so d <= 10 keeps
-// its natural allocation.
+// Kernels for d > 10, and FP32 kernels of any d, are held to 128 registers
+// (4 waves per SIMD) although some values then spill: at d = 20 that was
+// 4-21 % faster than 3 waves without spills, and the FP32 d = 10 mult kernel
+// (129 registers left to itself) 12 % faster; FP64 at d = 10 was 10-15 %
+// slower, so FP64 kernels with d <= 10 keep their natural allocation.
 template <typename T>
 constexpr int mult_k(int D) {
   return sizeof(T) == 8 ? (D <= 10 ? 4 : 2) : (D <= 10 ? 8 : 4);
@@ -77,28 +78,32 @@ __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * 
 //   3. evaluates only points that may be nonzero in the box, J at a time,
 //      over the union of their non-constant dimensions.
 // Every dropped term is exactly zero, so the sums equal a full evaluation.
-// X: [D][ldx] SoA, LI: point-major pairs, TB: [Npad/64][D][2] FP32 tile
-// boxes, A: [Npad] alpha*scale (zero padded), part: [G][ldx]. Block: kBlock*K
+// X: [D][ldx] SoA, WB: [ldx/(64*K)][D][2] boxes of the data groups, LI:
+// point-major pairs, TB: [Npad/64][D][2] FP32 tile boxes, A: [Npad]
+// alpha*scale (zero padded), part: [G][ldx]. Block: kBlock*K
 // data points (each wave 64*K consecutive ones) x grid chunk [y*chunk,
 // +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
 // kCount: also add the wave's executed hat steps and pairs to cnt[0], cnt[1].
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
-void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ LI,
-                 const float* __restrict__ TB, const uint32_t* __restrict__ PM,
-                 const T* __restrict__ A, double* __restrict__ part, int chunk, int jend_all,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
+void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
+                 const T* __restrict__ LI, const float* __restrict__ TB,
+                 const uint32_t* __restrict__ PM, const T* __restrict__ A,
+                 double* __restrict__ part, int chunk, int jend_all,
                  unsigned long long* __restrict__ cnt) {
   // a wave's 64*K points are contiguous (a small box in Morton order): point
-  // k of a lane is m0 + 64*k
+  // k of a lane is m0 + 64*k; the wave is data group g
   const int lane = threadIdx.x & 63;
-  const int64_t m0 = static_cast<int64_t>(blockIdx.x) * (kBlock * K) + (threadIdx.x >> 6) * (64 * K) + lane;
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) +
+                    __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int64_t m0 = g * (64 * K) + lane;
+  WaveBox<D, T> box;
+  load_box<D, T>(WB, g, box);
   T x[K][D];
 #pragma unroll
   for (int k = 0; k < K; ++k)
 #pragma unroll
     for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * 64];
-  WaveBox<D, T> box;
-  wave_box<D, D, K, T>(x, box);
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
@@ -188,15 +193,16 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // tile is then added, in FP64, to the wave's own row of LDS accumulators. At
 // the end the four rows are added in a fixed order. Block: data segment [blockIdx.x*seg, +seg) x
 // grid chunk [blockIdx.y*kChunkT, +kChunkT), the segment split into four
-// contiguous wave shares. part: [S][ldp].
+// contiguous wave shares (seg and the shares are multiples of 64*K, so every
+// group of 64*K points is one of WB's data groups). part: [S][ldp].
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 ? 4 : 1)))
-void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
-                  const T* __restrict__ LI, const float* __restrict__ TB,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
+void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
+                  const T* __restrict__ Y, const T* __restrict__ LI, const float* __restrict__ TB,
                   const uint32_t* __restrict__ PM, double* __restrict__ part, int64_t ldp,
                   int jend_all, int64_t seg, int64_t mend, unsigned long long* __restrict__ cnt) {
   __shared__ double accs[kBlock / 64][kChunkT];
-  const int wave = threadIdx.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   const int jb = blockIdx.y * kChunkT;
   const int je = min(jb + kChunkT, jend_all);
@@ -211,6 +217,8 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
   double* acc = accs[wave];
   unsigned long long n_steps = 0, n_pairs = 0;
   for (int64_t m0 = wb; m0 < we; m0 += 64 * K) {
+    WaveBox<D, T> box;
+    load_box<D, T>(WB, m0 / (64 * K), box);
     T x[K][D], y[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -219,8 +227,6 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ Y,
       for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m];
       y[k] = (m < we) ? Y[m] : T(0);
     }
-    WaveBox<D, T> box;
-    wave_box<D, D, K, T>(x, box);
     // the chunk's tiles (kChunkT / kTG <= 64), one lane per tile
     static_assert(kChunkT / kTG <= 64, "one window of tiles per chunk");
     unsigned long long tmask =
@@ -366,18 +372,18 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 // kSkip: test for constant dimensions (ModLinear); Linear grids have none.
 // kCount: the work-counting instantiation (cnt: 4 counters).
 template <typename T, int D, bool kSkip, bool kCount = false>
-void launch_mult_D(const T* X, int64_t ldx, const T* LI, const float* TB, const uint32_t* PM,
-                   const T* A, double* part, int chunk, int jend, int G, int64_t Mpad,
-                   unsigned long long* cnt, hipStream_t s) {
+void launch_mult_D(const T* X, int64_t ldx, const T* WB, const T* LI, const float* TB,
+                   const uint32_t* PM, const T* A, double* part, int chunk, int jend, int G,
+                   int64_t Mpad, unsigned long long* cnt, hipStream_t s) {
   constexpr int K = mult_k<T>(D);
   dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
   clear_launch_error();
   hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock), 0,
-                     s, X, ldx, LI, TB, PM, A, part, chunk, jend, cnt);
+                     s, X, ldx, WB, LI, TB, PM, A, part, chunk, jend, cnt);
 }
 
 template <typename T, int D, bool kSkip, bool kCount = false>
-void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const float* TB,
+void launch_multT_D(const T* X, int64_t ldx, const T* WB, const T* Y, const T* LI, const float* TB,
                     const uint32_t* PM, double* part, int64_t ldp, int jend, int64_t seg,
                     int64_t mend, int S, unsigned long long* cnt, hipStream_t s) {
   constexpr int K = multT_k<T>(D);
@@ -386,16 +392,17 @@ void launch_multT_D(const T* X, int64_t ldx, const T* Y, const T* LI, const floa
   dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>((jend + kChunkT - 1) / kChunkT));
   clear_launch_error();
   hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock),
-                     0, s, X, ldx, Y, LI, TB, PM, part, ldp, jend, seg, mend, cnt);
+                     0, s, X, ldx, WB, Y, LI, TB, PM, part, ldp, jend, seg, mend, cnt);
 }
 
 template <typename T>
-using MultFn = void (*)(const T*, int64_t, const T*, const float*, const uint32_t*, const T*,
-                        double*, int, int, int, int64_t, unsigned long long*, hipStream_t);
+using MultFn = void (*)(const T*, int64_t, const T*, const T*, const float*, const uint32_t*,
+                        const T*, double*, int, int, int, int64_t, unsigned long long*,
+                        hipStream_t);
 template <typename T>
-using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*, const uint32_t*,
-                         double*, int64_t, int, int64_t, int64_t, int, unsigned long long*,
-                         hipStream_t);
+using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const float*,
+                         const uint32_t*, double*, int64_t, int, int64_t, int64_t, int,
+                         unsigned long long*, hipStream_t);
 
 template <typename T, bool kCount, int... Ds>
 struct Table {
@@ -438,12 +445,9 @@ int ceil_div(int64_t a, int64_t b) { return static_cast<int>((a + b - 1) / b); }
 // dimension) and pairs (J slots x K points x 64 lanes per group) to cnt[0],
 // cnt[1] (mult) or cnt[2], cnt[3] (multTranspose).
 template <typename T>
-using CountMultFn = void (*)(const T*, int64_t, const T*, const float*, const uint32_t*, const T*,
-                             double*, int, int, int, int64_t, unsigned long long*, hipStream_t);
+using CountMultFn = MultFn<T>;
 template <typename T>
-using CountMultTFn = void (*)(const T*, int64_t, const T*, const T*, const float*,
-                              const uint32_t*, double*, int64_t, int, int64_t, int64_t, int,
-                              unsigned long long*, hipStream_t);
+using CountMultTFn = MultTFn<T>;
 template <typename T>
 CountMultFn<T> count_mult_launcher(int D, bool skip);
 template <typename T>
