@@ -99,11 +99,16 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
   const int64_t m0 = g * (64 * K) + lane;
   WaveBox<D, T> box;
   load_box<D, T>(WB, g, box);
+  // wave-uniform base per dimension, per-lane 32-bit offset (the loads take
+  // the scalar-base form: no per-load address arithmetic)
+  const T* xg = X + g * (64 * K);
   T x[K][D];
 #pragma unroll
-  for (int k = 0; k < K; ++k)
+  for (int d = 0; d < D; ++d) {
+    const T* xd = xg + d * ldx;
 #pragma unroll
-    for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m0 + k * 64];
+    for (int k = 0; k < K; ++k) x[k][d] = xd[static_cast<unsigned>(k * 64 + lane)];
+  }
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
@@ -219,12 +224,18 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB
   for (int64_t m0 = wb; m0 < we; m0 += 64 * K) {
     WaveBox<D, T> box;
     load_box<D, T>(WB, m0 / (64 * K), box);
+    // wave-uniform base per dimension, per-lane 32-bit offset (scalar-base
+    // loads)
     T x[K][D], y[K];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const T* xd = X + d * ldx + m0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) x[k][d] = xd[static_cast<unsigned>(k * 64 + lane)];
+    }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int64_t m = m0 + k * 64 + lane;
-#pragma unroll
-      for (int d = 0; d < D; ++d) x[k][d] = X[d * ldx + m];
       y[k] = (m < we) ? Y[m] : T(0);
     }
     // the chunk's tiles (kChunkT / kTG <= 64), one lane per tile
