@@ -48,20 +48,35 @@ __device__ __forceinline__ double dpp(double old, double v) {
                                              Ctrl, RowMask, 0xf, false);
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
 }
+// Cross-lane move without an `old` operand: lanes without a source lane read
+// 0 (bound_ctrl); lanes of rows outside RowMask receive an unspecified value.
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ float dpp_z(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), Ctrl, RowMask, 0xf, true));
+}
+template <int Ctrl, int RowMask = 0xf>
+__device__ __forceinline__ double dpp_z(double v) {
+  const long long s = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp(static_cast<int>(s), Ctrl, RowMask, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(static_cast<int>(s >> 32), Ctrl, RowMask, 0xf, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned>(lo));
+}
 // Reduction over the 64 lanes; the result is complete in lane 63 only.
 // Steps: swap within pairs and halves of each quad (every lane then holds its
 // quad's value), shift right by 4 and 8 within each row of 16 (lane 15 of a
-// row holds the row's value), broadcast lane 15 into the next row (lanes 31
-// and 63 hold two rows each), broadcast lane 31 into rows 2-3 (lane 63).
-// The lanes outside a step's reach add 0.
+// row holds the row's value; lanes without a source add 0), broadcast lane 15
+// into the next row (lanes 31 and 63 hold two rows each), broadcast lane 31
+// into rows 2-3 (lane 63). The two broadcasts leave rows 0, 2 (then 0, 1)
+// with unspecified sums; neither is read again: step 5 reads lanes 15 and 47
+// before it writes, and step 6 reads lane 31, which step 5 completed.
 template <typename T>
 __device__ __forceinline__ T sum_to_lane63(T v) {
-  v += dpp<0xb1>(T(0), v);        // quad_perm [1,0,3,2]
-  v += dpp<0x4e>(T(0), v);        // quad_perm [2,3,0,1]
-  v += dpp<0x114>(T(0), v);       // row_shr:4
-  v += dpp<0x118>(T(0), v);       // row_shr:8
-  v += dpp<0x142, 0xa>(T(0), v);  // row_bcast:15 into rows 1, 3
-  v += dpp<0x143, 0xc>(T(0), v);  // row_bcast:31 into rows 2, 3
+  v += dpp_z<0xb1>(v);        // quad_perm [1,0,3,2]
+  v += dpp_z<0x4e>(v);        // quad_perm [2,3,0,1]
+  v += dpp_z<0x114>(v);       // row_shr:4
+  v += dpp_z<0x118>(v);       // row_shr:8
+  v += dpp_z<0x142, 0xa>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_z<0x143, 0xc>(v);  // row_bcast:31 into rows 2, 3
   return v;
 }
 // lane 63's value, moved to a scalar register
