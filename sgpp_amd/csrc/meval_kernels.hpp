@@ -69,6 +69,26 @@ constexpr int kChunkT = 1024;
 // whole row with one wide scalar load; tile t = points [64t, 64t+64)).
 __device__ __forceinline__ int64_t li_at(int64_t j, int d, int D) { return (j * D + d) * 2; }
 
+// The box for a tile test. FP64 boxes of up to 10 dimensions (2*D register
+// pairs) are re-read with scalar loads for each test rather than held across
+// the loops: held, they are spilled to vector lanes and read back with one
+// v_readlane per value. (Above 10 dimensions the kernels are held to 128
+// vector registers, and re-reading the box made them spill more vector
+// registers instead.) The empty asm makes g opaque, so the loads are not
+// hoisted out of the loop.
+template <int D, typename T>
+__device__ __forceinline__ WaveBox<D, T> test_box(const T* __restrict__ WB, int64_t g,
+                                                  const WaveBox<D, T>& held) {
+  if constexpr (sizeof(T) == 8 && D <= 10) {
+    asm volatile("" : "+s"(g));
+    WaveBox<D, T> b;
+    load_box<D, T>(WB, g, b);
+    return b;
+  } else {
+    return held;
+  }
+}
+
 // ---------------------------------------------------------------- mult
 // Lanes own data points (K per lane, coordinates in registers). The data are
 // stored in Morton order, so a wave's 64*K points fill a small box. For every
@@ -118,7 +138,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
   // windows of up to 64 tiles: one lane per tile tests the tile's box
   for (int w0 = jb; w0 < je; w0 += 64 * kTG) {
   unsigned long long tmask = tiles_may_hit<D, D, T>(
-      TB, w0 / kTG, min(64, (je - w0 + kTG - 1) / kTG), lane, box);
+      TB, w0 / kTG, min(64, (je - w0 + kTG - 1) / kTG), lane, test_box(WB, g, box));
   while (tmask) {
     const int j0 = w0 + __builtin_ctzll(tmask) * kTG;
     tmask &= tmask - 1;
@@ -127,7 +147,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
     // this lane's point's dimensions, loaded with the tile test's rows (the
     // empty asm uses it there, so the load is not sunk to its first use)
     const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;
-    unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
+    unsigned long long bal = tile_hits<D, D, T>(tile, lane, test_box(WB, g, box));
     if (kSkip) asm volatile("" ::"v"(pm));
     T acc[K];
 #pragma unroll
@@ -250,7 +270,7 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB
       // this lane's point's dimensions, loaded with the tile test's rows (the
       // empty asm uses it there, so the load is not sunk to its first use)
       const uint32_t pm = kSkip ? PM[j0 + lane] : 0u;
-      unsigned long long bal = tile_hits<D, D, T>(tile, lane, box);
+      unsigned long long bal = tile_hits<D, D, T>(tile, lane, test_box(WB, m0 / (64 * K), box));
       if (kSkip) asm volatile("" ::"v"(pm));
       // a grid point is taken at most once per (group, tile): its sum goes
       // to lane jj of tsum, and the tile is added to the wave's row once
