@@ -412,7 +412,7 @@ void DeviceMultiEval::run_mult(const T* X, const T* LI, const float* TB, const d
     // work (culling makes blocks cost very different amounts) across all CUs
     const int target = cfg_.target_blocks > 0 ? cfg_.target_blocks : kBlocksPerCU * num_cus_;
     G = std::max(1, ceil_div(target, bx));
-    G = std::max(G, ceil_div(ntiles, kMaxChunkTiles));
+    G = std::max(G, ceil_div(ntiles, max_chunk_tiles<T>()));
     G = std::min(G, std::max(1, ntiles / 4));  // at least 4 tiles per chunk
     // partials G x Mpad doubles within the budget
     G = std::min<int64_t>(G, std::max<int64_t>(1, kPartialBudget / (Mpad_ * sizeof(double))));

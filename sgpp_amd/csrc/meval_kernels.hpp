@@ -26,14 +26,25 @@ constexpr int kTM = 128;     // data points per step of the generic multTranspos
 constexpr int kMaxTemplD = 20;
 // Default launch sizes. Culled blocks cost very different amounts, so many
 // short blocks balance better: mult ~128 workgroups per CU and grid chunks of
-// at most 24 tiles; multTranspose ~256 workgroups per CU (each block covers
-// kChunkT grid points). Measured on the d=10 ModLinear (1e5 x 1e6) and d=8
-// Linear (3.2e4 x 5e5) workloads.
+// at most 16 tiles (FP32) or 24 (FP64: as fast per launch, fewer partials to
+// reduce); multTranspose up to ~1024 workgroups per CU (each block
+// covers kChunkT grid points; the data segments stop at one 64*K-point group
+// per wave). Measured on the d=10 ModLinear (1e5 x 1e6), d=8 Linear
+// (3.2e4 x 5e5) and d=20 Linear (1e6 x 1e7) workloads (on the first, FP32
+// mult with 16-tile chunks took 1.97 ms against 2.08 with 24 and 1.98 with
+// 12; on the last, multTranspose with 264 data segments took 312 ms against
+// 362 with 66).
 constexpr int kBlocksPerCU = 128;
-constexpr int kMaxChunkTiles = 24;
-constexpr int kBlocksPerCUT = 256;
+template <typename T>
+constexpr int max_chunk_tiles() {
+  return sizeof(T) == 4 ? 16 : 24;
+}
+constexpr int kBlocksPerCUT = 1024;
 // cap on the FP64 partials of one launch (G x Mpad or S x Npad doubles)
-constexpr int64_t kPartialBudget = int64_t(512) << 20;
+// (on the d=20 Linear workload, 1e6 x 1e7, 512 MiB held mult to 6 grid
+// chunks and multTranspose to 66 data segments; 2 GiB allows 26 and 268,
+// 13-14 % faster)
+constexpr int64_t kPartialBudget = int64_t(2048) << 20;
 // data rows past M kept in the padded data (a wave's last group of 64*K
 // points may start just below M)
 constexpr size_t kDataSlack = 1024;
