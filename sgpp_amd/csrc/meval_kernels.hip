@@ -270,9 +270,10 @@ void DeviceMultiEval::upload_group_boxes(const std::vector<T>& soa, void** dst) 
 
 DeviceMultiEval::~DeviceMultiEval() {
   if (stream_) (void)hipStreamSynchronize(stream_);
-  void* bufs[] = {X64_,     X32_,    WB64_,   WB32_,   LI64_,   LI32_,   TB64_, TB32_, scale_, perm_,
-                  pm_,      dperm_,  A_,      Y_,      part_, hin_,  hout_,  sub_desc_,
-                  sub_ti_,  sub_te_, sub_tw_, sub_chunk_};
+  void* bufs[] = {X64_,   X32_,   WB64_,  WB32_,  LI64_,    LI32_,   TB64_,
+                  TB32_,  scale_, perm_,  pm_,    dperm_,   A_,      Y_,
+                  part_,  hin_,   hout_,  sub_desc_, sub_ti_, sub_te_, sub_tw_,
+                  sub_chunk_};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   for (auto* pool : {&tm_mult_, &tm_multT_})

@@ -54,7 +54,7 @@ constexpr size_t kDataSlack = 1024;
 // Kernels for d > 10, and FP32 kernels of any d, are held to 128 registers
 // (4 waves per SIMD) although some values then spill: at d = 20 that was
 // 4-21 % faster than 3 waves without spills, and the FP32 d = 10 mult kernel
-// (129 registers left to itself) 12 % faster; FP64 at d = 10 was 10-15 %
+// (129 registers left to itself) 12 % faster; FP64 at d = 10 was 10-40 %
 // slower, so FP64 kernels with d <= 10 keep their natural allocation.
 template <typename T>
 constexpr int mult_k(int D) {
@@ -116,7 +116,8 @@ __device__ __forceinline__ WaveBox<D, T> test_box(const T* __restrict__ WB, int6
 // +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
 // kCount: also add the wave's executed hat steps and pairs to cnt[0], cnt[1].
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
 void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
                  const T* __restrict__ LI, const float* __restrict__ TB,
                  const uint32_t* __restrict__ PM, const T* __restrict__ A,
@@ -232,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void mult_kernel_generic(const T* __restric
 // contiguous wave shares (seg and the shares are multiples of 64*K, so every
 // group of 64*K points is one of WB's data groups). part: [S][ldp].
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
 void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
                   const T* __restrict__ Y, const T* __restrict__ LI, const float* __restrict__ TB,
                   const uint32_t* __restrict__ PM, double* __restrict__ part, int64_t ldp,
