@@ -113,7 +113,10 @@ __device__ __forceinline__ WaveBox<D, T> test_box(const T* __restrict__ WB, int6
 // point-major pairs, TB: [Npad/64][D][2] FP32 tile boxes, A: [Npad]
 // alpha*scale (zero padded), part: [G][ldx]. Block: kBlock*K
 // data points (each wave 64*K consecutive ones) x grid chunk [y*chunk,
-// +chunk). kSkip: test for constant (ModLinear level-1) dimensions.
+// +chunk). chunk_fastest: the grid is (chunks, data blocks) instead of
+// (data blocks, chunks), so a data block's chunks run one after the other and
+// its coordinates stay in the caches (the partials are the same either way).
+// kSkip: test for constant (ModLinear level-1) dimensions.
 // kCount: also add the wave's executed hat steps and pairs to cnt[0], cnt[1].
 template <int D, int K, int J, typename T, bool kSkip, bool kCount>
 __global__ __launch_bounds__(kBlock)
@@ -121,12 +124,14 @@ __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
 void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
                  const T* __restrict__ LI, const float* __restrict__ TB,
                  const uint32_t* __restrict__ PM, const T* __restrict__ A,
-                 double* __restrict__ part, int chunk, int jend_all,
+                 double* __restrict__ part, int chunk, int jend_all, int chunk_fastest,
                  unsigned long long* __restrict__ cnt) {
+  const unsigned bdata = chunk_fastest ? blockIdx.y : blockIdx.x;
+  const unsigned bchunk = chunk_fastest ? blockIdx.x : blockIdx.y;
   // a wave's 64*K points are contiguous (a small box in Morton order): point
   // k of a lane is m0 + 64*k; the wave is data group g
   const int lane = threadIdx.x & 63;
-  const int64_t g = static_cast<int64_t>(blockIdx.x) * (kBlock / 64) +
+  const int64_t g = static_cast<int64_t>(bdata) * (kBlock / 64) +
                     __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t m0 = g * (64 * K) + lane;
   WaveBox<D, T> box;
@@ -144,7 +149,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
   double accd[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) accd[k] = 0.0;
-  const int jb = blockIdx.y * chunk;
+  const int jb = static_cast<int>(bchunk) * chunk;
   const int je = min(jb + chunk, jend_all);
   unsigned long long n_steps = 0, n_pairs = 0;
   // windows of up to 64 tiles: one lane per tile tests the tile's box
@@ -185,7 +190,7 @@ void mult_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
     for (int k = 0; k < K; ++k) accd[k] += static_cast<double>(acc[k]);
   }
   }
-  double* out = part + static_cast<int64_t>(blockIdx.y) * ldx;
+  double* out = part + static_cast<int64_t>(bchunk) * ldx;
 #pragma unroll
   for (int k = 0; k < K; ++k) out[m0 + k * 64] = accd[k];
   if (kCount && lane == 0) {
@@ -418,12 +423,14 @@ __global__ void reduce_parts_kernel(const double* __restrict__ part, int64_t ld,
 template <typename T, int D, bool kSkip, bool kCount = false>
 void launch_mult_D(const T* X, int64_t ldx, const T* WB, const T* LI, const float* TB,
                    const uint32_t* PM, const T* A, double* part, int chunk, int jend, int G,
-                   int64_t Mpad, unsigned long long* cnt, hipStream_t s) {
+                   int64_t Mpad, int chunk_fastest, unsigned long long* cnt, hipStream_t s) {
   constexpr int K = mult_k<T>(D);
-  dim3 grid(static_cast<unsigned>(Mpad / (kBlock * K)), static_cast<unsigned>(G));
+  const unsigned bx = static_cast<unsigned>(Mpad / (kBlock * K));
+  const dim3 grid = chunk_fastest ? dim3(static_cast<unsigned>(G), bx)
+                                  : dim3(bx, static_cast<unsigned>(G));
   clear_launch_error();
   hipLaunchKernelGGL((mult_kernel<D, K, mult_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock), 0,
-                     s, X, ldx, WB, LI, TB, PM, A, part, chunk, jend, cnt);
+                     s, X, ldx, WB, LI, TB, PM, A, part, chunk, jend, chunk_fastest, cnt);
 }
 
 template <typename T, int D, bool kSkip, bool kCount = false>
@@ -441,7 +448,7 @@ void launch_multT_D(const T* X, int64_t ldx, const T* WB, const T* Y, const T* L
 
 template <typename T>
 using MultFn = void (*)(const T*, int64_t, const T*, const T*, const float*, const uint32_t*,
-                        const T*, double*, int, int, int, int64_t, unsigned long long*,
+                        const T*, double*, int, int, int, int64_t, int, unsigned long long*,
                         hipStream_t);
 template <typename T>
 using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const float*,
