@@ -393,8 +393,8 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
 // chunks fastest was 8 % faster in FP32, on a d=8 regular grid (dense hits)
 // 3-8 % slower, and on a d=20 grid with 1e7 points (long blocks) 5 % slower.
 // So both orders are timed once per grid, on zero weights (the culled work
-// does not depend on them), and the faster kept; a launch longer than
-// kOrderProbeMs keeps data blocks fastest without the second run.
+// does not depend on them), and the faster kept; a first launch longer than
+// kOrderProbeMs keeps data blocks fastest without further runs.
 constexpr float kOrderProbeMs = 20.0f;
 
 template <typename T>
@@ -416,14 +416,21 @@ void DeviceMultiEval::choose_mult_order(const T* X, const T* LI, const float* TB
     check_hip(hipMemsetAsync(dA, 0, N_ * sizeof(double), stream_), "hipMemsetAsync(probe)");
     check_hip(hipEventCreate(&ev[0]), "hipEventCreate");
     check_hip(hipEventCreate(&ev[1]), "hipEventCreate");
+    // per order a first launch (a long first one ends the probe: large
+    // problems pay one extra launch), then the timed one
+    auto timed = [&]() {
+      run_mult<T>(X, LI, TB, dA, dR, ev);
+      check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+      float t = 0.0f;
+      check_hip(hipEventElapsedTime(&t, ev[0], ev[1]), "hipEventElapsedTime");
+      return t;
+    };
     float ms[2] = {0.0f, 0.0f};
     for (int o = 0; o < 2; ++o) {
       cf = o;
-      run_mult<T>(X, LI, TB, dA, dR, nullptr);  // warm-up
-      run_mult<T>(X, LI, TB, dA, dR, ev);
-      check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
-      check_hip(hipEventElapsedTime(&ms[o], ev[0], ev[1]), "hipEventElapsedTime");
+      ms[o] = timed();
       if (o == 0 && ms[0] > kOrderProbeMs) break;
+      ms[o] = timed();
     }
     cf = (ms[0] <= kOrderProbeMs && ms[1] < ms[0]) ? 1 : 0;
   } catch (...) {
