@@ -272,7 +272,8 @@ typedef struct sgpp_hip_server_config {
 enum { SGPP_HIP_SERVE_AUTO = 0, SGPP_HIP_SERVE_STREAMING = 1, SGPP_HIP_SERVE_SUBSPACE = 2 };
 
 /* A tuned shape: grid points per work item and the measured seconds per
- * (query x grid point). */
+ * (query x grid point). A model with more than 256 x chunk grid points uses
+ * larger items, so that it has at most 256 grid chunks per query. */
 typedef struct sgpp_hip_server_shape {
   int chunk;
   double cost;
