@@ -167,9 +167,11 @@ class DeviceMultiEval {
   // data-group boxes of the templated kernels (set at construction)
   template <typename T>
   void upload_group_boxes(const std::vector<T>& soa, void** dst);
-  // block order of the templated mult kernel per precision (index 0 FP64,
-  // 1 FP32): 1 = grid chunks fastest; timed when the grid is set
+  // block order of the templated mult / multTranspose kernels per precision
+  // (index 0 FP64, 1 FP32): 1 = grid chunks fastest; timed when the grid is
+  // set
   int mult_cf_[2] = {0, 0};
+  int multT_cf_[2] = {0, 0};
   template <typename T>
   void choose_mult_order(const T* X, const T* LI, const float* TB);
   template <typename T>

@@ -385,7 +385,8 @@ void DeviceMultiEval::set_grid(const uint32_t* level, const uint32_t* index, siz
   }
 }
 
-// Block order of the templated mult kernel: data blocks fastest (consecutive
+// Block order of the templated mult and multTranspose kernels (the same
+// choice for each): data blocks fastest (consecutive
 // workgroups share a grid chunk, whose rows stay in the caches) or grid
 // chunks fastest (a data block's coordinates stay in the caches). Both give
 // the same partials, bit for bit. Which is faster depends on how densely the
@@ -399,8 +400,8 @@ constexpr float kOrderProbeMs = 20.0f;
 
 template <typename T>
 void DeviceMultiEval::choose_mult_order(const T* X, const T* LI, const float* TB) {
-  int& cf = mult_cf_[sizeof(T) == sizeof(double) ? 0 : 1];
-  cf = 0;
+  const int ix = sizeof(T) == sizeof(double) ? 0 : 1;
+  mult_cf_[ix] = multT_cf_[ix] = 0;
   double* dA = nullptr;
   double* dR = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -414,27 +415,36 @@ void DeviceMultiEval::choose_mult_order(const T* X, const T* LI, const float* TB
     check_hip(hipMalloc(reinterpret_cast<void**>(&dA), N_ * sizeof(double)), "hipMalloc(probe)");
     check_hip(hipMalloc(reinterpret_cast<void**>(&dR), M_ * sizeof(double)), "hipMalloc(probe)");
     check_hip(hipMemsetAsync(dA, 0, N_ * sizeof(double), stream_), "hipMemsetAsync(probe)");
+    check_hip(hipMemsetAsync(dR, 0, M_ * sizeof(double), stream_), "hipMemsetAsync(probe)");
     check_hip(hipEventCreate(&ev[0]), "hipEventCreate");
     check_hip(hipEventCreate(&ev[1]), "hipEventCreate");
-    // per order a first launch (a long first one ends the probe: large
-    // problems pay one extra launch), then the timed one
-    auto timed = [&]() {
-      run_mult<T>(X, LI, TB, dA, dR, ev);
-      check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
-      float t = 0.0f;
-      check_hip(hipEventElapsedTime(&t, ev[0], ev[1]), "hipEventElapsedTime");
-      return t;
-    };
-    float ms[2] = {0.0f, 0.0f};
-    for (int o = 0; o < 2; ++o) {
-      cf = o;
-      ms[o] = timed();
-      if (o == 0 && ms[0] > kOrderProbeMs) break;
-      ms[o] = timed();
+    // per kernel and order a first launch (a long first one ends that
+    // kernel's probe: large problems pay one extra launch), then the timed
+    // one; multTranspose runs on y = dR (zero: set above, and mult's output
+    // on zero weights) and writes dA
+    for (int kern = 0; kern < 2; ++kern) {
+      int& cf = kern == 0 ? mult_cf_[ix] : multT_cf_[ix];
+      auto timed = [&]() {
+        if (kern == 0)
+          run_mult<T>(X, LI, TB, dA, dR, ev);
+        else
+          run_multT<T>(X, LI, TB, dR, dA, ev);
+        check_hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+        float t = 0.0f;
+        check_hip(hipEventElapsedTime(&t, ev[0], ev[1]), "hipEventElapsedTime");
+        return t;
+      };
+      float ms[2] = {0.0f, 0.0f};
+      for (int o = 0; o < 2; ++o) {
+        cf = o;
+        ms[o] = timed();
+        if (o == 0 && ms[0] > kOrderProbeMs) break;
+        ms[o] = timed();
+      }
+      cf = (ms[0] <= kOrderProbeMs && ms[1] < ms[0]) ? 1 : 0;
     }
-    cf = (ms[0] <= kOrderProbeMs && ms[1] < ms[0]) ? 1 : 0;
   } catch (...) {
-    cf = 0;
+    mult_cf_[ix] = multT_cf_[ix] = 0;
     release();
     throw;
   }
@@ -564,9 +574,11 @@ void DeviceMultiEval::run_multT(const T* X, const T* LI, const float* TB, const 
     const bool skip = basis_ == Basis::ModLinear;
     MultTFn<T> f = count_ ? count_multT_launcher<T>(static_cast<int>(D_), skip)
                           : DimTable<T>::multT(static_cast<int>(D_), skip);
+    // grid chunks fastest only where the segments fit the grid's y extent
+    const int cf = multT_cf_[sizeof(T) == sizeof(double) ? 0 : 1] && S <= 65535 ? 1 : 0;
     f(X, static_cast<int64_t>(Mpad_), group_boxes<T>(), static_cast<const T*>(Y_), LI, TB,
       static_cast<const uint32_t*>(pm_), P, static_cast<int64_t>(Npad_), jend, seg, mend, S,
-      count_, s);
+      cf, count_, s);
   } else {
     dim3 grid(static_cast<unsigned>(bx), static_cast<unsigned>(S));
     clear_launch_error();

@@ -243,15 +243,19 @@ __attribute__((amdgpu_waves_per_eu(D > 10 || sizeof(T) == 4 ? 4 : 1)))
 void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB,
                   const T* __restrict__ Y, const T* __restrict__ LI, const float* __restrict__ TB,
                   const uint32_t* __restrict__ PM, double* __restrict__ part, int64_t ldp,
-                  int jend_all, int64_t seg, int64_t mend, unsigned long long* __restrict__ cnt) {
+                  int jend_all, int64_t seg, int64_t mend, int chunk_fastest,
+                  unsigned long long* __restrict__ cnt) {
   __shared__ double accs[kBlock / 64][kChunkT];
+  // chunk_fastest: the grid is (chunks, segments), as in mult_kernel
+  const unsigned bseg = chunk_fastest ? blockIdx.y : blockIdx.x;
+  const unsigned bchunk = chunk_fastest ? blockIdx.x : blockIdx.y;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int jb = blockIdx.y * kChunkT;
+  const int jb = static_cast<int>(bchunk) * kChunkT;
   const int je = min(jb + kChunkT, jend_all);
   for (int i = threadIdx.x; i < (kBlock / 64) * kChunkT; i += kBlock) (&accs[0][0])[i] = 0.0;
   __syncthreads();
-  const int64_t mb = static_cast<int64_t>(blockIdx.x) * seg;
+  const int64_t mb = static_cast<int64_t>(bseg) * seg;
   const int64_t me = min(mb + seg, mend);
   // per-wave share, a multiple of 64*K; points past the share weigh 0
   const int64_t share = ((me - mb) / (kBlock / 64) + 64 * K - 1) / (64 * K) * (64 * K);
@@ -321,7 +325,7 @@ void multT_kernel(const T* __restrict__ X, int64_t ldx, const T* __restrict__ WB
     }
   }
   __syncthreads();
-  double* out = part + static_cast<int64_t>(blockIdx.x) * ldp;
+  double* out = part + static_cast<int64_t>(bseg) * ldp;
   for (int i = threadIdx.x; i < je - jb; i += kBlock) {
     double s = accs[0][i];
 #pragma unroll
@@ -436,14 +440,19 @@ void launch_mult_D(const T* X, int64_t ldx, const T* WB, const T* LI, const floa
 template <typename T, int D, bool kSkip, bool kCount = false>
 void launch_multT_D(const T* X, int64_t ldx, const T* WB, const T* Y, const T* LI, const float* TB,
                     const uint32_t* PM, double* part, int64_t ldp, int jend, int64_t seg,
-                    int64_t mend, int S, unsigned long long* cnt, hipStream_t s) {
+                    int64_t mend, int S, int chunk_fastest, unsigned long long* cnt,
+                    hipStream_t s) {
   constexpr int K = multT_k<T>(D);
-  // data segments vary fastest: consecutive blocks share one grid chunk
-  // (its rows stay hot in every XCD's caches) and read different data
-  dim3 grid(static_cast<unsigned>(S), static_cast<unsigned>((jend + kChunkT - 1) / kChunkT));
+  // data segments fastest: consecutive blocks share one grid chunk (its rows
+  // stay hot in every XCD's caches) and read different data; or chunks
+  // fastest: consecutive blocks share a data segment
+  const unsigned nch = static_cast<unsigned>((jend + kChunkT - 1) / kChunkT);
+  const dim3 grid = chunk_fastest ? dim3(nch, static_cast<unsigned>(S))
+                                  : dim3(static_cast<unsigned>(S), nch);
   clear_launch_error();
   hipLaunchKernelGGL((multT_kernel<D, K, multT_j<T>(D), T, kSkip, kCount>), grid, dim3(kBlock),
-                     0, s, X, ldx, WB, Y, LI, TB, PM, part, ldp, jend, seg, mend, cnt);
+                     0, s, X, ldx, WB, Y, LI, TB, PM, part, ldp, jend, seg, mend, chunk_fastest,
+                     cnt);
 }
 
 template <typename T>
@@ -452,7 +461,7 @@ using MultFn = void (*)(const T*, int64_t, const T*, const T*, const float*, con
                         hipStream_t);
 template <typename T>
 using MultTFn = void (*)(const T*, int64_t, const T*, const T*, const T*, const float*,
-                         const uint32_t*, double*, int64_t, int, int64_t, int64_t, int,
+                         const uint32_t*, double*, int64_t, int, int64_t, int64_t, int, int,
                          unsigned long long*, hipStream_t);
 
 template <typename T, bool kCount, int... Ds>
